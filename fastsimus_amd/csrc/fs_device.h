@@ -1,0 +1,135 @@
+// Device-side building blocks shared by the gfx950 kernels.
+//
+// Green's function of one (scatterer s, element e, sub-element j) term at
+// frequency f (reference math: _pfield_math.py:71-247, simus.py:263-333):
+//
+//   g(f) = amp * exp(-att * f) * exp(i 2 pi f t)
+//   amp  = obliquity(theta) * directivity(theta) / sqrt(r_c) / n_sub
+//   att  = alpha_dB / (20 / ln 10) * 1e-4 * r_c         (Np per Hz)
+//   t    = r_c / c                                       (s)
+//   r_c  = max(r, c / (2 fc)),  theta = asin(dx / r) - theta_e
+//
+// r and the phase f * t (mod 1) are evaluated in FP64; everything that only
+// scales amplitude is FP32. Phases are carried in revolutions so that the
+// range reduction is an exact FP64 fract().
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fastsimus.h"
+
+namespace fs {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr double kNeperToDb = 8.6858896380650365;  // 20 / ln(10)
+
+struct cf {
+  float x, y;
+};
+__device__ __forceinline__ cf cmul(cf a, cf b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ void cmac(cf& acc, cf a, cf b) {
+  acc.x = fmaf(a.x, b.x, acc.x);
+  acc.x = fmaf(-a.y, b.y, acc.x);
+  acc.y = fmaf(a.x, b.y, acc.y);
+  acc.y = fmaf(a.y, b.x, acc.y);
+}
+
+// Scalars of one problem, copied by value into every kernel.
+struct Problem {
+  int S, E, NS, NF, NT;
+  int fulldir, baffle;
+  double f0, df, inv_c, min_dist;
+  float att_coef;       // alpha_dB / (20/ln10) * 1e-4   (Np / (m Hz))
+  float kappa;          // pi * fc * seg / c  (centre-frequency sinc scale)
+  float kappa_per_hz;   // pi * seg / c       (per-frequency sinc scale)
+  float baffle_b;
+  float inv_nsub;
+  double radius, apex;
+  float corr;
+  const float *x, *z, *rc;
+  const double *sub_x, *sub_z, *ecos, *esin;
+  const double *delay, *apod;
+  const double *pulse_re, *pulse_im, *probe;
+};
+
+// Per-(s, e, j) geometry that does not depend on frequency.
+struct Geo {
+  double t;     // r_c / c
+  float amp;    // obliquity * directivity (centre frequency unless fulldir) / sqrt(r_c) / n_sub
+  float att;    // Np / Hz
+  float sint;   // sin(theta), used by the per-frequency directivity
+};
+
+__device__ __forceinline__ float sinc_rad(float a) {
+  // sin(a) / a with the removable singularity at 0 (numpy.sinc(a / pi)).
+  return (a == 0.0f) ? 1.0f : sinf(a) / a;
+}
+
+__device__ __forceinline__ bool out_of_field(const Problem& p, float xs, float zs) {
+  bool out = zs < 0.0f;
+  if (isfinite(p.radius)) {
+    double X = xs, Z = (double)zs + p.apex;
+    out = out || (X * X + Z * Z <= p.radius * p.radius);
+  }
+  return out;
+}
+
+__device__ __forceinline__ Geo geometry(const Problem& p, double xs, double zs, int e, int j) {
+  const int idx = e * p.NS + j;
+  const double dx = xs - p.sub_x[idx];
+  const double dz = zs - p.sub_z[idx];
+  const double r = sqrt(fma(dx, dx, dz * dz));
+  const double rcl = fmax(r, p.min_dist);
+  const float rinv = 1.0f / (float)(r + 1e-16);
+  const float sa = (float)(dx + 1e-16) * rinv;  // sin(asin(u)) with the reference's 1e-16 guard
+  const float ca = fabsf((float)dz) * rinv;     // cos(asin(u)) = |dz| / r
+  const float ce = (float)p.ecos[e], se = (float)p.esin[e];
+  const float st = sa * ce - ca * se;           // sin(asin(u) - theta_e)
+  const float ct = ca * ce + sa * se;           // cos(asin(u) - theta_e)
+  float obl;
+  if (p.baffle == FS_BAFFLE_SOFT) {
+    obl = ct;
+  } else if (p.baffle == FS_BAFFLE_RIGID) {
+    obl = 1.0f;
+  } else {
+    obl = ct / (ct + p.baffle_b);
+  }
+  if (!(ct > 0.0f)) obl = 1e-16f;               // |theta| >= pi/2
+  const float rcf = (float)rcl;
+  float amp = obl * rsqrtf(rcf) * p.inv_nsub;
+  if (!p.fulldir) amp *= sinc_rad(p.kappa * st);
+  Geo g;
+  g.t = rcl * p.inv_c;
+  g.amp = amp;
+  g.att = p.att_coef * rcf;
+  g.sint = st;
+  return g;
+}
+
+// exp(-att f) exp(i 2 pi frac(f t)) * amp, with the phase reduced in FP64.
+__device__ __forceinline__ cf green_at(const Geo& g, double f) {
+  const double nu = f * g.t;
+  const float ph = (float)(nu - floor(nu));
+  float s, c;
+  sincospif(2.0f * ph, &s, &c);
+  const float m = (g.att != 0.0f) ? g.amp * expf(-g.att * (float)f) : g.amp;
+  return {m * c, m * s};
+}
+
+// Per-step multiplier exp((-att + i 2 pi t) df).
+__device__ __forceinline__ cf green_step(const Geo& g, double df) {
+  const double nu = df * g.t;
+  const float ph = (float)(nu - floor(nu));
+  float s, c;
+  sincospif(2.0f * ph, &s, &c);
+  const float m = (g.att != 0.0f) ? expf(-g.att * (float)df) : 1.0f;
+  return {m * c, m * s};
+}
+
+// Per-frequency directivity sinc(k_f seg/2 sin(theta) / pi).
+__device__ __forceinline__ float dir_at(const Problem& p, const Geo& g, double f) {
+  return sinc_rad(p.kappa_per_hz * (float)f * g.sint);
+}
+
+}  // namespace fs

@@ -1,0 +1,365 @@
+// Plain FP32 SIMUS spectrum: one transmit at a time, TX then RX.
+//
+// Reference algorithm per selected frequency f (simus.py:263-333,
+// _simus_strategies.py:23-77):
+//   P(f, s)  = PP(f) * sum_e G(f, e, s) A(f, e),   0 where s is out of field
+//   X(f, e)  = corr * probe(f) * sum_s rc_s P(f, s) G(f, e, s)
+// with G the sub-element mean of the Green's function and
+// A(f, e) = w_e exp(i 2 pi f tau_e).
+//
+// Layout / mapping (MI355X, wave64):
+//  * frequencies are processed in blocks of 32; at the start of every block
+//    each (s, e, j) term is reseeded from an FP64 phase f * r_c / c mod 1 and
+//    advanced by an FP32 complex recurrence inside the block;
+//  * k_tx: lane = scatterer, elements are walked uniformly by the wave, so the
+//    transmit table A(f, e) is a wave-uniform (scalar) load; the 32 P(f, s)
+//    accumulators live in VGPRs; the result Q = rc PP P goes to HBM as
+//    Q[block][s][32];
+//  * k_rx: lane = element, scatterers are walked uniformly, so Q(f, s) is a
+//    scalar load; the 32 X(f, e) accumulators live in VGPRs; every wave writes a
+//    private partial sum (no atomics) and k_reduce adds them in a fixed order,
+//    so results are bitwise reproducible.
+#include <algorithm>
+
+#include "fs_device.h"
+
+namespace fs {
+
+constexpr int FB = 32;  // frequencies per reseed block
+
+// A[t][e][k] = w exp(i 2 pi f_k tau), k < nfp (padded to a multiple of FB).
+__global__ void k_tx_table(Problem p, int nfp, cf* __restrict__ A) {
+  const long total = (long)p.NT * p.E * nfp;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(i % nfp);
+    const long te = i / nfp;
+    const double tau = p.delay[te];
+    const double w = p.apod[te];
+    const double f = p.f0 + (double)k * p.df;
+    const double nu = f * tau;
+    const float ph = (float)(nu - floor(nu));
+    float s, c;
+    sincospif(2.0f * ph, &s, &c);
+    A[i] = {(float)w * c, (float)w * s};
+  }
+}
+
+// PP(f_k) = pulse * probe, and probe, padded with zeros beyond NF.
+__global__ void k_spectra(Problem p, int nfp, cf* __restrict__ pp, float* __restrict__ pr) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nfp) return;
+  if (k < p.NF) {
+    const double q = p.probe[k];
+    pp[k] = {(float)(p.pulse_re[k] * q), (float)(p.pulse_im[k] * q)};
+    pr[k] = (float)q;
+  } else {
+    pp[k] = {0.f, 0.f};
+    pr[k] = 0.f;
+  }
+}
+
+// MODE 0 (simus): Q[blockIdx.y][s][k] = rc_s PP_k P_k(s).
+// MODE 1 (pfield): pw[fb][s] = sum_{k in block} |PP_k P_k(s)|^2 (reference _freq_step_body, _pfield_strategies.py:21-50).
+template <int NSUB, bool FULLDIR, int MODE>
+__global__ __launch_bounds__(256) void k_tx(Problem p, int tx, int nfp, int fb0, const cf* __restrict__ A,
+                                            const cf* __restrict__ pp, cf* __restrict__ Q, float* __restrict__ pw) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int fb = fb0 + blockIdx.y;
+  const int k0 = fb * FB;
+  const bool valid = s < p.S;
+  const float xs = valid ? p.x[s] : 0.f;
+  const float zs = valid ? p.z[s] : 1e-2f;
+  const float rcs = (MODE == 0 && valid) ? p.rc[s] : 1.f;
+  const bool out = !valid || out_of_field(p, xs, zs) || rcs == 0.f;
+
+  cf P[FB];
+#pragma unroll
+  for (int k = 0; k < FB; ++k) P[k] = {0.f, 0.f};
+
+  const double fstart = p.f0 + (double)k0 * p.df;
+  const cf* At = A + (size_t)tx * p.E * nfp + k0;
+  for (int e = 0; e < p.E; ++e) {
+    cf g[NSUB], q[NSUB];
+    Geo geo[NSUB];
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+      geo[j] = geometry(p, xs, zs, e, j);
+      g[j] = green_at(geo[j], fstart);
+      q[j] = green_step(geo[j], p.df);
+    }
+    const cf* Ae = At + (size_t)e * nfp;
+#pragma unroll
+    for (int k = 0; k < FB; ++k) {
+      cf G = {0.f, 0.f};
+#pragma unroll
+      for (int j = 0; j < NSUB; ++j) {
+        if constexpr (FULLDIR) {
+          const float d = dir_at(p, geo[j], fstart + (double)k * p.df);
+          G.x = fmaf(g[j].x, d, G.x);
+          G.y = fmaf(g[j].y, d, G.y);
+        } else {
+          G.x += g[j].x;
+          G.y += g[j].y;
+        }
+        g[j] = cmul(g[j], q[j]);
+      }
+      cmac(P[k], G, Ae[k]);
+    }
+  }
+  if (!valid) return;
+  if constexpr (MODE == 0) {
+    cf* Qs = Q + ((size_t)blockIdx.y * p.S + s) * FB;
+#pragma unroll
+    for (int k = 0; k < FB; ++k) {
+      cf v = out ? cf{0.f, 0.f} : cmul(P[k], pp[k0 + k]);
+      Qs[k] = {v.x * rcs, v.y * rcs};
+    }
+  } else {
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < FB; ++k) {
+      const cf v = cmul(P[k], pp[k0 + k]);  // pp is zero beyond NF
+      acc = fmaf(v.x, v.x, fmaf(v.y, v.y, acc));
+    }
+    pw[(size_t)fb * p.S + s] = out ? 0.f : acc;
+  }
+}
+
+// out[s] = sqrt(corr * sum_fb pw[fb][s])
+__global__ void k_pfield_reduce(const float* __restrict__ pw, int nfb, int S, float corr, float* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  float acc = 0.f;
+  for (int b = 0; b < nfb; ++b) acc += pw[(size_t)b * S + s];
+  out[s] = sqrtf(acc * corr);
+}
+
+// part[(chunk * 4 + wave)][k - k_begin][e], k_begin = first frequency of the block group
+template <int NSUB, bool FULLDIR>
+__global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_per_wave, const cf* __restrict__ Q,
+                                            cf* __restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int e = blockIdx.x * 64 + lane;
+  const int fb = fb0 + blockIdx.y;
+  const int k0 = fb * FB;
+  const int chunk = blockIdx.z;
+  const bool valid = e < p.E;
+  const int ee = valid ? e : 0;
+
+  cf X[FB];
+#pragma unroll
+  for (int k = 0; k < FB; ++k) X[k] = {0.f, 0.f};
+
+  const double fstart = p.f0 + (double)k0 * p.df;
+  const int sb = (chunk * 4 + wave) * s_per_wave;
+  const int se = min(sb + s_per_wave, p.S);
+  const cf* Qb = Q + (size_t)blockIdx.y * p.S * FB;
+  for (int s = sb; s < se; ++s) {
+    const cf* Qs = Qb + (size_t)s * FB;
+    if (p.rc[s] == 0.f) continue;  // wave-uniform: the scatterer contributes nothing
+    const float xs = p.x[s], zs = p.z[s];
+    if (out_of_field(p, xs, zs)) continue;
+    cf g[NSUB], q[NSUB];
+    Geo geo[NSUB];
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+      geo[j] = geometry(p, xs, zs, ee, j);
+      g[j] = green_at(geo[j], fstart);
+      q[j] = green_step(geo[j], p.df);
+    }
+#pragma unroll
+    for (int k = 0; k < FB; ++k) {
+      cf G = {0.f, 0.f};
+#pragma unroll
+      for (int j = 0; j < NSUB; ++j) {
+        if constexpr (FULLDIR) {
+          const float d = dir_at(p, geo[j], fstart + (double)k * p.df);
+          G.x = fmaf(g[j].x, d, G.x);
+          G.y = fmaf(g[j].y, d, G.y);
+        } else {
+          G.x += g[j].x;
+          G.y += g[j].y;
+        }
+        g[j] = cmul(g[j], q[j]);
+      }
+      cmac(X[k], G, Qs[k]);
+    }
+  }
+  if (!valid) return;
+  cf* out = part + ((size_t)(chunk * 4 + wave) * nbuf + blockIdx.y * FB) * p.E + e;
+#pragma unroll
+  for (int k = 0; k < FB; ++k) out[(size_t)k * p.E] = X[k];
+}
+
+// spect[tx][k][e] = corr * probe(k) * sum_w part[w][k][e]   (k < NF)
+__global__ void k_reduce(Problem p, int nbuf, int nparts, int k_begin, int k_end, const cf* __restrict__ part,
+                         const float* __restrict__ pr, int tx, cf* __restrict__ spect) {
+  const long n = (long)(k_end - k_begin) * p.E;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int k = k_begin + (int)(i / p.E);
+    const int e = (int)(i % p.E);
+    if (k >= p.NF) continue;
+    float sx = 0.f, sy = 0.f;
+    for (int w = 0; w < nparts; ++w) {
+      const cf v = part[((size_t)w * nbuf + (k - k_begin)) * p.E + e];
+      sx += v.x;
+      sy += v.y;
+    }
+    const float s = p.corr * pr[k];
+    spect[((size_t)tx * p.NF + k) * p.E + e] = {sx * s, sy * s};
+  }
+}
+
+// ----------------------------------------------------------------------------- host side
+struct Fp32Plan {
+  int nfp, nfb, fbg, chunks, s_per_wave, nparts;
+  size_t off_A, off_pp, off_pr, off_Q, off_part, total;
+};
+
+static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+Fp32Plan fp32_plan(const Problem& p) {
+  Fp32Plan P{};
+  P.nfb = (p.NF + FB - 1) / FB;
+  P.nfp = P.nfb * FB;
+  // Q buffer holds fbg frequency blocks for all scatterers: cap it near 256 MiB.
+  const size_t per_block = (size_t)p.S * FB * sizeof(cf);
+  P.fbg = (int)std::max<size_t>(1, std::min<size_t>((size_t)P.nfb, (256u << 20) / std::max<size_t>(per_block, 1)));
+  const int etiles = (p.E + 63) / 64;
+  // enough waves per RX launch to fill 256 CUs several times over
+  const long want_waves = 8192;
+  const long waves_per_chunk = (long)etiles * P.fbg * 4;
+  int chunks = (int)std::max<long>(1, want_waves / std::max<long>(waves_per_chunk, 1));
+  const int max_chunks = std::max(1, (p.S + 4 * 16 - 1) / (4 * 16));  // at least 16 scatterers per wave
+  chunks = std::min(chunks, max_chunks);
+  P.chunks = chunks;
+  P.s_per_wave = (p.S + chunks * 4 - 1) / (chunks * 4);
+  P.nparts = chunks * 4;
+  size_t off = 0;
+  P.off_A = off;    off = align_up(off + (size_t)p.NT * p.E * P.nfp * sizeof(cf));
+  P.off_pp = off;   off = align_up(off + (size_t)P.nfp * sizeof(cf));
+  P.off_pr = off;   off = align_up(off + (size_t)P.nfp * sizeof(float));
+  P.off_Q = off;    off = align_up(off + per_block * P.fbg);
+  P.off_part = off; off = align_up(off + (size_t)P.nparts * P.fbg * FB * p.E * sizeof(cf));
+  P.total = off;
+  return P;
+}
+
+template <int NSUB, bool FD>
+static void launch_tx(const Problem& p, int tx, const Fp32Plan& P, int fb0, int nb, const cf* A, const cf* pp, cf* Q,
+                      hipStream_t st) {
+  dim3 grid((p.S + 255) / 256, nb);
+  hipLaunchKernelGGL((k_tx<NSUB, FD, 0>), grid, dim3(256), 0, st, p, tx, P.nfp, fb0, A, pp, Q, (float*)nullptr);
+}
+template <int NSUB, bool FD>
+static void launch_rx(const Problem& p, const Fp32Plan& P, int fb0, int nb, const cf* Q, cf* part, hipStream_t st) {
+  dim3 grid((p.E + 63) / 64, nb, P.chunks);
+  hipLaunchKernelGGL((k_rx<NSUB, FD>), grid, dim3(256), 0, st, p, P.fbg * FB, fb0, P.s_per_wave, Q, part);
+}
+
+template <int NSUB>
+static int dispatch_fd(const Problem& p, int tx, const Fp32Plan& P, int fb0, int nb, const cf* A, const cf* pp, cf* Q,
+                       cf* part, hipStream_t st) {
+  if (p.fulldir) {
+    launch_tx<NSUB, true>(p, tx, P, fb0, nb, A, pp, Q, st);
+    launch_rx<NSUB, true>(p, P, fb0, nb, Q, part, st);
+  } else {
+    launch_tx<NSUB, false>(p, tx, P, fb0, nb, A, pp, Q, st);
+    launch_rx<NSUB, false>(p, P, fb0, nb, Q, part, st);
+  }
+  return FS_OK;
+}
+
+int fp32_spectrum(const Problem& p, cf* spect, char* ws, hipStream_t st) {
+  const Fp32Plan P = fp32_plan(p);
+  cf* A = (cf*)(ws + P.off_A);
+  cf* pp = (cf*)(ws + P.off_pp);
+  float* pr = (float*)(ws + P.off_pr);
+  cf* Q = (cf*)(ws + P.off_Q);
+  cf* part = (cf*)(ws + P.off_part);
+  {
+    const long n = (long)p.NT * p.E * P.nfp;
+    const int blocks = (int)std::min<long>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_tx_table, dim3(blocks), dim3(256), 0, st, p, P.nfp, A);
+    hipLaunchKernelGGL(k_spectra, dim3((P.nfp + 255) / 256), dim3(256), 0, st, p, P.nfp, pp, pr);
+  }
+  for (int tx = 0; tx < p.NT; ++tx) {
+    for (int fb0 = 0; fb0 < P.nfb; fb0 += P.fbg) {
+      const int nb = std::min(P.fbg, P.nfb - fb0);
+      int rc;
+      switch (p.NS) {
+        case 1: rc = dispatch_fd<1>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
+        case 2: rc = dispatch_fd<2>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
+        case 3: rc = dispatch_fd<3>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
+        case 4: rc = dispatch_fd<4>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
+        default: return FS_ERR_UNSUPPORTED;
+      }
+      if (rc != FS_OK) return rc;
+      const int kb = fb0 * FB, ke = (fb0 + nb) * FB;
+      const long n = (long)(ke - kb) * p.E;
+      const int blocks = (int)std::min<long>((n + 255) / 256, 65536);
+      hipLaunchKernelGGL(k_reduce, dim3(blocks), dim3(256), 0, st, p, P.fbg * FB, P.nparts, kb, ke, part, pr, tx, spect);
+    }
+  }
+  return hipGetLastError() == hipSuccess ? FS_OK : FS_ERR_LAUNCH;
+}
+
+size_t fp32_workspace(const Problem& p) { return fp32_plan(p).total; }
+
+// ----------------------------------------------------------------------------- pfield
+struct PfieldLayout {
+  int nfb, nfp;
+  size_t off_A, off_pp, off_pr, off_pw, total;
+};
+
+static PfieldLayout pfield_layout(const Problem& p) {
+  PfieldLayout L{};
+  L.nfb = (p.NF + FB - 1) / FB;
+  L.nfp = L.nfb * FB;
+  size_t off = 0;
+  L.off_A = off;  off = align_up(off + (size_t)p.E * L.nfp * sizeof(cf));
+  L.off_pp = off; off = align_up(off + (size_t)L.nfp * sizeof(cf));
+  L.off_pr = off; off = align_up(off + (size_t)L.nfp * sizeof(float));
+  L.off_pw = off; off = align_up(off + (size_t)L.nfb * std::max(p.S, 1) * sizeof(float));
+  L.total = off;
+  return L;
+}
+
+size_t pfield_workspace(const Problem& p) { return pfield_layout(p).total; }
+
+template <int NSUB>
+static void launch_pfield(const Problem& p, const PfieldLayout& L, const cf* A, const cf* pp, float* pw,
+                          hipStream_t st) {
+  dim3 grid((p.S + 255) / 256, L.nfb);
+  if (p.fulldir)
+    hipLaunchKernelGGL((k_tx<NSUB, true, 1>), grid, dim3(256), 0, st, p, 0, L.nfp, 0, A, pp, (cf*)nullptr, pw);
+  else
+    hipLaunchKernelGGL((k_tx<NSUB, false, 1>), grid, dim3(256), 0, st, p, 0, L.nfp, 0, A, pp, (cf*)nullptr, pw);
+}
+
+int pfield_fp32(const Problem& p, float* out, char* ws, hipStream_t st) {
+  const PfieldLayout L = pfield_layout(p);
+  cf* A = (cf*)(ws + L.off_A);
+  cf* pp = (cf*)(ws + L.off_pp);
+  float* pr = (float*)(ws + L.off_pr);
+  float* pw = (float*)(ws + L.off_pw);
+  Problem q = p;
+  q.NT = 1;
+  {
+    const long n = (long)q.E * L.nfp;
+    hipLaunchKernelGGL(k_tx_table, dim3((int)std::min<long>((n + 255) / 256, 65536)), dim3(256), 0, st, q, L.nfp, A);
+    hipLaunchKernelGGL(k_spectra, dim3((L.nfp + 255) / 256), dim3(256), 0, st, q, L.nfp, pp, pr);
+  }
+  switch (q.NS) {
+    case 1: launch_pfield<1>(q, L, A, pp, pw, st); break;
+    case 2: launch_pfield<2>(q, L, A, pp, pw, st); break;
+    case 3: launch_pfield<3>(q, L, A, pp, pw, st); break;
+    case 4: launch_pfield<4>(q, L, A, pp, pw, st); break;
+    default: return FS_ERR_UNSUPPORTED;
+  }
+  hipLaunchKernelGGL(k_pfield_reduce, dim3((q.S + 255) / 256), dim3(256), 0, st, pw, L.nfb, q.S, q.corr, out);
+  return hipGetLastError() == hipSuccess ? FS_OK : FS_ERR_LAUNCH;
+}
+
+}  // namespace fs

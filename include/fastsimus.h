@@ -1,0 +1,136 @@
+/*
+ * fastsimus.h - C ABI of the MI355X (gfx950) SIMUS hot path.
+ *
+ * Every pointer is a device pointer owned by the caller; every call is
+ * stream-ordered on the given hipStream_t (passed as void* so that the header
+ * needs no HIP include). Inputs are never written. Outputs are fully
+ * overwritten. Functions return FS_OK (0) or a status code; fs_status_string
+ * names it. The library allocates no device memory: scratch is the caller's
+ * `workspace` of at least fs_simus_workspace_bytes() bytes.
+ *
+ * Reference interfaces replaced (charlesbmi/FastSIMUS):
+ *   fs_simus_spectrum     <- simus_cuda(scatterers, rc, params, plan, medium,
+ *                            delays_clean, tx_apodization)
+ *                            src/fast_simus/kernels/cuda_simus.py:232-321
+ *                            (kernel simus_fused_kernel, simus_fused.cu:62-332),
+ *                            times plan.correction_factor (simus.py:490-491)
+ *   fs_spectrum_to_rf     <- _irfft_and_threshold, src/fast_simus/simus.py:336-367
+ *   fs_simus_adjoint      <- no kernel in the reference; differentiates the
+ *                            algebra of _simus_freq_outer_scan
+ *                            (src/fast_simus/_simus_strategies.py:152-210)
+ *   fs_rf_adjoint         <- reverse of _irfft_and_threshold (simus.py:336-367)
+ *   fs_pfield             <- pfield_metal / _freq_outer_python
+ *                            (src/fast_simus/kernels/metal_pfield.py:89-219,
+ *                             src/fast_simus/_pfield_strategies.py:53-88)
+ */
+#ifndef FASTSIMUS_H
+#define FASTSIMUS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum fs_status {
+  FS_OK = 0,
+  FS_ERR_INVALID = 1,      /* bad sizes / null pointers            -> ValueError   */
+  FS_ERR_UNSUPPORTED = 2,  /* option not implemented by this method -> NotImplementedError */
+  FS_ERR_WORKSPACE = 3,    /* workspace smaller than required       -> RuntimeError */
+  FS_ERR_LAUNCH = 4,       /* HIP launch / runtime error            -> RuntimeError */
+  FS_ERR_RESOURCE = 5      /* size beyond a kernel resource limit   -> RuntimeError */
+};
+
+enum fs_method {
+  FS_METHOD_FP32 = 0,      /* per-transmit TX/RX contractions on FP32 VALU, FP64 reseed every 32 steps */
+  FS_METHOD_MFMA_BF16X3 = 1 /* all transmits through one multistatic matrix per frequency, split-BF16 MFMA */
+};
+
+enum fs_baffle { FS_BAFFLE_SOFT = 0, FS_BAFFLE_RIGID = 1, FS_BAFFLE_IMPEDANCE = 2 };
+
+/* One SIMUS problem: scatterers x probe x transmits x selected band. */
+typedef struct fs_simus_desc {
+  int32_t n_scat;   /* S  */
+  int32_t n_elem;   /* E  */
+  int32_t n_sub;    /* sub-elements per element */
+  int32_t n_freq;   /* selected frequencies */
+  int32_t n_tx;     /* transmits */
+  int32_t full_frequency_directivity; /* 0: centre-frequency sinc, 1: sinc at every frequency */
+  int32_t baffle_kind;                /* fs_baffle */
+  int32_t pad0;
+  /* scatterers, float32 [S] */
+  const float* scat_x;
+  const float* scat_z;
+  const float* rc;
+  /* sub-element centroids, float64 [E * n_sub] (element-major), element angle cos/sin [E] */
+  const double* sub_x;
+  const double* sub_z;
+  const double* elem_cos;
+  const double* elem_sin;
+  /* transmits, float64 [n_tx * E], NaN already replaced (delay 0, apodization 0) */
+  const double* tx_delay;
+  const double* tx_apod;
+  /* spectra at the selected frequencies, float64 [n_freq] */
+  const double* pulse_re;
+  const double* pulse_im;
+  const double* probe;
+  /* scalars */
+  double f0;                /* first selected frequency (Hz) */
+  double df;                /* frequency step (Hz) */
+  double speed_of_sound;    /* m/s */
+  double attenuation;       /* dB/cm/MHz */
+  double freq_center;       /* Hz */
+  double seg_length;        /* sub-element length (m) */
+  double radius;            /* convex radius (m), +inf for linear */
+  double apex_offset;       /* m */
+  double baffle_impedance;  /* used when baffle_kind == FS_BAFFLE_IMPEDANCE */
+  double correction;        /* plan correction factor (df * element width) */
+} fs_simus_desc;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int fs_version(void);
+const char* fs_status_string(int status);
+
+/* Bytes of workspace the method needs for this problem. */
+size_t fs_simus_workspace_bytes(const fs_simus_desc* d, int method);
+
+/* Selected-band spectrum of every transmit, already multiplied by d->correction:
+ *   spect[t][k][e] (interleaved re,im float32), t < n_tx, k < n_freq, e < n_elem.
+ * Replaces simus_cuda(...) * plan.correction_factor. */
+int fs_simus_spectrum(const fs_simus_desc* d, int method, float* spect, void* workspace, size_t workspace_bytes,
+                      void* stream);
+
+/* Place the band into the full spectrum and compute gated RF per transmit:
+ *   spectrum_full[t][i][e] (complex64, i < n_ff), zero outside [idx0, idx0 + n_freq);
+ *   rf[t][n][e] = gate(irfft(conj(full), n_fft)[n]), n < (n_fft + 1) / 2.
+ * workspace: >= fs_rf_workspace_bytes(n_tx). May be called with spectrum_full == NULL.
+ * rf_ungated, when not NULL, receives the pre-gate RF (needed by fs_rf_adjoint). */
+size_t fs_rf_workspace_bytes(int n_tx);
+int fs_spectrum_to_rf(const float* spect, int n_tx, int n_freq, int n_elem, int n_ff, int idx0, int n_fft,
+                      float* spectrum_full, float* rf, float* rf_ungated, void* workspace, size_t workspace_bytes,
+                      void* stream);
+
+/* Reverse of fs_spectrum_to_rf: given dL/d(rf) of the gated RF and the pre-gate RF, writes
+ * dL/d(spect) for the selected band (interleaved complex float32, [n_tx][n_freq][n_elem]);
+ * the complex cotangent convention is g = dL/dRe + i dL/dIm. */
+int fs_rf_adjoint(const float* rf_ungated, const float* d_rf, int n_tx, int n_freq, int n_elem, int idx0, int n_fft,
+                  float* d_spect, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Gradients of L through fs_simus_spectrum (method FP32 algebra) given dL/d(spect) (the corrected
+ * selected spectrum, interleaved complex): d_rc, d_x, d_z [S] float32 (overwritten),
+ * d_c [1] float64 (overwritten). Green's functions are recomputed, not stored. */
+size_t fs_simus_adjoint_workspace_bytes(const fs_simus_desc* d);
+int fs_simus_adjoint(const fs_simus_desc* d, const float* d_spect, float* d_rc, float* d_x, float* d_z, double* d_c,
+                     void* workspace, size_t workspace_bytes, void* stream);
+
+/* RMS pressure field of one transmit on n_scat points (scat_x/scat_z reused as the grid; rc unused):
+ *   out[p] = sqrt(correction * sum_k |pulse_k probe_k sum_{e,j} g A|^2), 0 where out of field. */
+size_t fs_pfield_workspace_bytes(const fs_simus_desc* d);
+int fs_pfield(const fs_simus_desc* d, float* out, void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTSIMUS_H */
