@@ -30,6 +30,8 @@ EXPORTS = (
     "fs_simus_adjoint",
     "fs_pfield_workspace_bytes",
     "fs_pfield",
+    "fs_set_kernel_timing",
+    "fs_kernel_time_ms",
 )
 
 
@@ -85,8 +87,19 @@ def load() -> C.CDLL:
     lib.fs_pfield_workspace_bytes.argtypes = [D]
     lib.fs_pfield.restype = C.c_int
     lib.fs_pfield.argtypes = [D, P, P, C.c_size_t, P]
+    lib.fs_set_kernel_timing.restype = C.c_int
+    lib.fs_set_kernel_timing.argtypes = [C.c_int]
+    lib.fs_kernel_time_ms.restype = C.c_double
+    lib.fs_kernel_time_ms.argtypes = [C.POINTER(C.c_int)]
     _lib = lib
     return lib
+
+
+def kernel_time_ms() -> tuple[float, int]:
+    """Summed duration (ms) and count of the timed dominant-kernel launches since the last call."""
+    n = C.c_int(0)
+    ms = load().fs_kernel_time_ms(C.byref(n))
+    return float(ms), int(n.value)
 
 
 def check(status: int, what: str) -> None:

@@ -132,4 +132,7 @@ __device__ __forceinline__ float dir_at(const Problem& p, const Geo& g, double f
   return sinc_rad(p.kappa_per_hz * (float)f * g.sint);
 }
 
+int timing_start(hipStream_t st);
+void timing_stop(int slot, hipStream_t st);
+
 }  // namespace fs

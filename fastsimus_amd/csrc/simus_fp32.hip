@@ -250,12 +250,16 @@ template <int NSUB, bool FD>
 static void launch_tx(const Problem& p, int tx, const Fp32Plan& P, int fb0, int nb, const cf* A, const cf* pp, cf* Q,
                       hipStream_t st) {
   dim3 grid((p.S + 255) / 256, nb);
+  const int slot = timing_start(st);
   hipLaunchKernelGGL((k_tx<NSUB, FD, 0>), grid, dim3(256), 0, st, p, tx, P.nfp, fb0, A, pp, Q, (float*)nullptr);
+  timing_stop(slot, st);
 }
 template <int NSUB, bool FD>
 static void launch_rx(const Problem& p, const Fp32Plan& P, int fb0, int nb, const cf* Q, cf* part, hipStream_t st) {
   dim3 grid((p.E + 63) / 64, nb, P.chunks);
+  const int slot = timing_start(st);
   hipLaunchKernelGGL((k_rx<NSUB, FD>), grid, dim3(256), 0, st, p, P.fbg * FB, fb0, P.s_per_wave, Q, part);
+  timing_stop(slot, st);
 }
 
 template <int NSUB>

@@ -1,0 +1,446 @@
+// Batched SIMUS through the multistatic matrix on the gfx950 matrix cores.
+//
+// Every transmit t shares the Green's functions G(f, e, s). Writing the
+// reference algorithm (simus.py:263-333, _simus_strategies.py:23-77) per
+// frequency f with transmit vector A_t(f, e') = w exp(i 2 pi f tau):
+//
+//   X_t(f, e) = corr probe(f) sum_s rc_s G(f,e,s) [PP(f) sum_e' G(f,e',s) A_t(f,e')]
+//             = corr probe(f) PP(f) sum_e' K_f[e, e'] A_t(f, e'),
+//   K_f[e, e'] = sum_{s in field} rc_s G(f, e, s) G(f, e', s)        (complex symmetric, E x E)
+//
+// so the scatterer sum is done once for all transmits. K_f is a rank-S
+// product H^T diag(sign rc) H with H = sqrt|rc| G, evaluated with
+// v_mfma_f32_16x16x32_bf16 in split precision: every operand is hi + lo
+// bf16 and hi*hi + hi*lo + lo*hi is accumulated in FP32.
+//
+// Complex -> real: the 32-deep MFMA contraction index runs over
+// (real part, imaginary part) of 16 scatterers; one fragment of element e
+// holds [Re H(e, s0..s3), Im H(e, s0..s3)] for the lane group's 4 scatterers.
+// With that fragment F(e) as both operands:
+//   Re K = (Re H | -Im H)(e) . sign (Re H | Im H)(e')
+//   Im K = (Im H |  Re H)(e) . sign (Re H | Im H)(e')
+// i.e. the A operands are the fragment with its upper half negated or its
+// halves swapped, and the B operand is the fragment with the sign of rc
+// applied: no transposes, one LDS layout for both sides.
+//
+// Workgroup = 4 waves, one block of FBK frequencies and one chunk of
+// scatterers. Per tile of 16 scatterers:
+//   1. every thread evaluates G for 4 scatterers x EPT elements at the FBK
+//      frequencies (FP64 r and phase seed, FP32 recurrence inside the block)
+//      and writes split-bf16 fragments to LDS;
+//   2. every wave runs its share of the upper-triangular 16x16 tiles of K_f
+//      (2x2 block split of the tile grid) with accumulators in registers.
+// Partial K per scatterer chunk goes to HBM; k_msm_apply sums the chunks in a
+// fixed order and applies every transmit (E x E x n_tx per frequency, FP32).
+#include <algorithm>
+
+#include "fs_device.h"
+
+namespace fs {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int MST = 16;  // scatterers per tile (32-deep contraction: re + im)
+
+template <int ET, int FBK>
+struct FragLds {
+  static constexpr int per_k = 2 * 2 * ET * 64;  // (T1|T2) x (hi|lo) x tiles x lanes
+  static constexpr size_t bytes = (size_t)FBK * per_k * 16;
+  __device__ static __forceinline__ int idx(int k, int t, int hl, int tile, int lane) {
+    return (((k * 2 + t) * 2 + hl) * ET + tile) * 64 + lane;
+  }
+};
+
+__device__ __forceinline__ void split_pair(float a, float b, unsigned& hi, unsigned& lo) {
+  const bf16x2 h = __builtin_convertvector((floatx2){a, b}, bf16x2);
+  const unsigned hb = __builtin_bit_cast(unsigned, h);
+  const float ah = __uint_as_float(hb << 16), bh = __uint_as_float(hb & 0xffff0000u);
+  const bf16x2 l = __builtin_convertvector((floatx2){a - ah, b - bh}, bf16x2);
+  hi = hb;
+  lo = __builtin_bit_cast(unsigned, l);
+}
+
+__device__ __forceinline__ floatx4 mma(u32x4 a, u32x4 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                 0, 0);
+}
+
+// ---------------------------------------------------------------- tile jobs
+// Upper-triangular tile pairs (a <= b) of an ET x ET grid, split into four
+// groups: L x L, U x U, L0 x U, L1 x U with L = [0, ET/2), U = [ET/2, ET),
+// L0/L1 the two halves of L.
+template <int ET>
+struct Jobs {
+  static constexpr int H = ET / 2;
+  static constexpr int H0 = H / 2;
+  __host__ __device__ static constexpr int count(int w) {
+    return w == 0 ? H * (H + 1) / 2 : w == 1 ? (ET - H) * (ET - H + 1) / 2 : w == 2 ? H0 * (ET - H) : (H - H0) * (ET - H);
+  }
+  __host__ __device__ static constexpr void pair(int w, int j, int& a, int& b) {
+    if (w == 0 || w == 1) {
+      const int base = (w == 0) ? 0 : H;
+      const int n = (w == 0) ? H : ET - H;
+      int r = 0, jj = j;
+      while (jj >= n - r) {
+        jj -= n - r;
+        ++r;
+      }
+      a = base + r;
+      b = base + r + jj;
+    } else {
+      const int r0 = (w == 2) ? 0 : H0;
+      const int nb = ET - H;
+      a = r0 + j / nb;
+      b = H + j % nb;
+    }
+  }
+  __host__ __device__ static constexpr int row_lo(int w) { return w == 0 ? 0 : w == 1 ? H : w == 2 ? 0 : H0; }
+  __host__ __device__ static constexpr int row_hi(int w) { return w == 0 ? H : w == 1 ? ET : w == 2 ? H0 : H; }
+  __host__ __device__ static constexpr int col_lo(int w) { return w == 0 ? 0 : H; }
+  __host__ __device__ static constexpr int col_hi(int w) { return w == 0 ? H : ET; }
+  static constexpr int max_count() {
+    int m = 0;
+    for (int w = 0; w < 4; ++w) m = count(w) > m ? count(w) : m;
+    return m;
+  }
+};
+
+// ---------------------------------------------------------------- G fragments
+template <int NSUB, int EPT, int FBK, bool FULLDIR, int ET>
+__device__ __forceinline__ void gen_fragments(const Problem& p, int s0, int s_end, int fb, u32x4* lds) {
+  using L = FragLds<ET, FBK>;
+  const int g = threadIdx.x >> 6;   // scatterer quad of the tile
+  const int el = threadIdx.x & 63;  // element group
+  float w[4];
+  unsigned m01 = 0u, m23 = 0u;
+  double xs[4], zs[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int s = s0 + 4 * g + i;
+    const bool valid = s < s_end;
+    const float x = valid ? p.x[s] : 0.f;
+    const float z = valid ? p.z[s] : 1e-2f;
+    const float r = valid ? p.rc[s] : 0.f;
+    const bool live = valid && r != 0.f && !out_of_field(p, x, z);
+    w[i] = live ? sqrtf(fabsf(r)) : 0.f;
+    const unsigned sb = (r < 0.f) ? 1u : 0u;
+    if (i < 2)
+      m01 |= sb << (15 + 16 * i);
+    else
+      m23 |= sb << (15 + 16 * (i - 2));
+    xs[i] = x;
+    zs[i] = z;
+  }
+  const double fstart = p.f0 + (double)(fb * FBK) * p.df;
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int e = el * EPT + u;
+    const bool ev = e < p.E;
+    const int ee = ev ? e : 0;
+    cf gs[4][NSUB], qs[4][NSUB];
+    float st[4][NSUB];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int j = 0; j < NSUB; ++j) {
+        const Geo geo = geometry(p, xs[i], zs[i], ee, j);
+        gs[i][j] = green_at(geo, fstart);
+        qs[i][j] = green_step(geo, p.df);
+        st[i][j] = geo.sint;
+      }
+    }
+    const int tile = e >> 4;
+    const int lane = (e & 15) + 16 * g;
+#pragma unroll
+    for (int k = 0; k < FBK; ++k) {
+      float re[4], im[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        cf G = {0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < NSUB; ++j) {
+          if constexpr (FULLDIR) {
+            const float d = sinc_rad(p.kappa_per_hz * (float)(fstart + (double)k * p.df) * st[i][j]);
+            G.x = fmaf(gs[i][j].x, d, G.x);
+            G.y = fmaf(gs[i][j].y, d, G.y);
+          } else {
+            G.x += gs[i][j].x;
+            G.y += gs[i][j].y;
+          }
+          if (k + 1 < FBK) gs[i][j] = cmul(gs[i][j], qs[i][j]);
+        }
+        const float wi = ev ? w[i] : 0.f;
+        re[i] = G.x * wi;
+        im[i] = G.y * wi;
+      }
+      unsigned h0, h1, h2, h3, l0, l1, l2, l3;
+      split_pair(re[0], re[1], h0, l0);
+      split_pair(re[2], re[3], h1, l1);
+      split_pair(im[0], im[1], h2, l2);
+      split_pair(im[2], im[3], h3, l3);
+      const u32x4 hi = {h0, h1, h2, h3}, lo = {l0, l1, l2, l3};
+      if (tile < ET) {
+        lds[L::idx(k, 0, 0, tile, lane)] = hi;
+        lds[L::idx(k, 0, 1, tile, lane)] = lo;
+        const u32x4 msk = {m01, m23, m01, m23};
+        lds[L::idx(k, 1, 0, tile, lane)] = hi ^ msk;
+        lds[L::idx(k, 1, 1, tile, lane)] = lo ^ msk;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- MFMA tiles of one wave
+template <int ET, int FBK, int W>
+struct WaveTiles {
+  using J = Jobs<ET>;
+  static constexpr int NJ = J::count(W);
+  static constexpr int R0 = J::row_lo(W), R1 = J::row_hi(W), C0 = J::col_lo(W), C1 = J::col_hi(W);
+  static constexpr int NR = R1 - R0, NC = C1 - C0;
+  floatx4 acc[FBK][NJ > 0 ? NJ : 1][2];
+
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int k = 0; k < FBK; ++k)
+#pragma unroll
+      for (int j = 0; j < (NJ > 0 ? NJ : 1); ++j) acc[k][j][0] = acc[k][j][1] = (floatx4){0.f, 0.f, 0.f, 0.f};
+  }
+
+  __device__ __forceinline__ void step(const u32x4* lds) {
+    using L = FragLds<ET, FBK>;
+    const int lane = threadIdx.x & 63;
+    const u32x4 nim = {0u, 0u, 0x80008000u, 0x80008000u};
+#pragma unroll
+    for (int k = 0; k < FBK; ++k) {
+      u32x4 arh[NR > 0 ? NR : 1], arl[NR > 0 ? NR : 1], aih[NR > 0 ? NR : 1], ail[NR > 0 ? NR : 1];
+      u32x4 bh[NC > 0 ? NC : 1], bl[NC > 0 ? NC : 1];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const u32x4 h = lds[L::idx(k, 0, 0, R0 + r, lane)];
+        const u32x4 l = lds[L::idx(k, 0, 1, R0 + r, lane)];
+        arh[r] = h ^ nim;
+        arl[r] = l ^ nim;
+        aih[r] = (u32x4){h.z, h.w, h.x, h.y};
+        ail[r] = (u32x4){l.z, l.w, l.x, l.y};
+      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        bh[c] = lds[L::idx(k, 1, 0, C0 + c, lane)];
+        bl[c] = lds[L::idx(k, 1, 1, C0 + c, lane)];
+      }
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        int a = 0, b = 0;
+        J::pair(W, j, a, b);
+        const int r = a - R0, c = b - C0;
+        floatx4 kr = acc[k][j][0], ki = acc[k][j][1];
+        kr = mma(arh[r], bh[c], kr);
+        ki = mma(aih[r], bh[c], ki);
+        kr = mma(arh[r], bl[c], kr);
+        ki = mma(aih[r], bl[c], ki);
+        kr = mma(arl[r], bh[c], kr);
+        ki = mma(ail[r], bh[c], ki);
+        acc[k][j][0] = kr;
+        acc[k][j][1] = ki;
+      }
+    }
+  }
+
+  // Kpart[chunk][kglob][e][e'] (complex), D layout: lane -> (row 4*(lane/16)+i, col lane%16)
+  __device__ __forceinline__ void store(cf* kpart, int E_pad, int kbase) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < FBK; ++k) {
+      cf* Kk = kpart + (size_t)(kbase + k) * E_pad * E_pad;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        int a = 0, b = 0;
+        J::pair(W, j, a, b);
+        const int col = 16 * b + (lane & 15);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = 16 * a + 4 * (lane >> 4) + i;
+          Kk[(size_t)row * E_pad + col] = {acc[k][j][0][i], acc[k][j][1][i]};
+        }
+      }
+    }
+  }
+};
+
+template <int NSUB, int ET, int FBK, bool FULLDIR, int W>
+__device__ __forceinline__ void msm_wave(const Problem& p, int sb, int se, int fb, u32x4* lds, cf* kpart) {
+  constexpr int EPT = (ET * 16) / 64;
+  WaveTiles<ET, FBK, W> t;
+  t.zero();
+  for (int s0 = sb; s0 < se; s0 += MST) {
+    gen_fragments<NSUB, EPT, FBK, FULLDIR, ET>(p, s0, se, fb, lds);
+    __syncthreads();
+    t.step(lds);
+    __syncthreads();
+  }
+  t.store(kpart, ET * 16, fb * FBK);
+}
+
+template <int NSUB, int ET, int FBK, bool FULLDIR>
+__global__ __launch_bounds__(256) void k_msm(Problem p, int s_per_chunk, int nfpad, cf* __restrict__ kpart) {
+  extern __shared__ u32x4 lds_frag[];
+  const int fb = blockIdx.x;
+  const int chunk = blockIdx.y;
+  const int sb = chunk * s_per_chunk;
+  const int se = min(sb + s_per_chunk, p.S);
+  cf* kp = kpart + (size_t)chunk * nfpad * (ET * 16) * (ET * 16);
+  switch (threadIdx.x >> 6) {
+    case 0: msm_wave<NSUB, ET, FBK, FULLDIR, 0>(p, sb, se, fb, lds_frag, kp); break;
+    case 1: msm_wave<NSUB, ET, FBK, FULLDIR, 1>(p, sb, se, fb, lds_frag, kp); break;
+    case 2: msm_wave<NSUB, ET, FBK, FULLDIR, 2>(p, sb, se, fb, lds_frag, kp); break;
+    default: msm_wave<NSUB, ET, FBK, FULLDIR, 3>(p, sb, se, fb, lds_frag, kp); break;
+  }
+}
+
+// ---------------------------------------------------------------- apply all transmits
+// out[t][k][e] = corr probe_k PP_k sum_e' K_k[e][e'] A_t(k, e'),  K_k = sum_c Kpart[c][k] (upper tiles, symmetric)
+constexpr int AP_CH = 32;  // e' per LDS chunk
+
+__global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, int nchunks, int nfpad,
+                                                   const cf* __restrict__ kpart, cf* __restrict__ out) {
+  __shared__ cf Ks[128][AP_CH + 1];
+  __shared__ cf As[AP_CH][2 * 40 + 1];
+  const int k = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int e = tid & 127;
+  const int tg = tid >> 7;  // transmit parity
+  const double f = p.f0 + (double)k * p.df;
+  constexpr int MAXT = 40;  // transmits per pass per thread parity (80 per pass)
+  for (int t0 = 0; t0 < p.NT; t0 += 2 * MAXT) {
+    cf acc[MAXT];
+#pragma unroll
+    for (int i = 0; i < MAXT; ++i) acc[i] = {0.f, 0.f};
+    for (int c0 = 0; c0 < p.E; c0 += AP_CH) {
+      __syncthreads();
+      for (int q = tid; q < 128 * AP_CH; q += 256) {
+        const int r = q / AP_CH, cc = q % AP_CH;
+        const int col = c0 + cc;
+        cf v = {0.f, 0.f};
+        if (r < p.E && col < p.E) {
+          const int ra = ((r >> 4) <= (col >> 4)) ? r : col;
+          const int cb = ((r >> 4) <= (col >> 4)) ? col : r;
+          float sx = 0.f, sy = 0.f;
+          for (int c = 0; c < nchunks; ++c) {
+            const cf w = kpart[(((size_t)c * nfpad + k) * E_pad + ra) * E_pad + cb];
+            sx += w.x;
+            sy += w.y;
+          }
+          v = {sx, sy};
+        }
+        Ks[r][cc] = v;
+      }
+      for (int q = tid; q < AP_CH * 2 * MAXT; q += 256) {
+        const int cc = q % AP_CH, ti = q / AP_CH;
+        const int t = t0 + ti, col = c0 + cc;
+        cf a = {0.f, 0.f};
+        if (t < p.NT && col < p.E) {
+          const double tau = p.delay[(size_t)t * p.E + col];
+          const double wgt = p.apod[(size_t)t * p.E + col];
+          const double nu = f * tau;
+          const float ph = (float)(nu - floor(nu));
+          float s, c;
+          sincospif(2.0f * ph, &s, &c);
+          a = {(float)wgt * c, (float)wgt * s};
+        }
+        As[cc][ti] = a;
+      }
+      __syncthreads();
+      if (e < p.E) {
+        const int cmax = min(AP_CH, p.E - c0);
+        for (int cc = 0; cc < cmax; ++cc) {
+          const cf kv = Ks[e][cc];
+#pragma unroll
+          for (int i = 0; i < MAXT; ++i) cmac(acc[i], kv, As[cc][2 * i + tg]);
+        }
+      }
+    }
+    if (e < p.E) {
+      const double q = p.probe[k];
+      const float sre = (float)(p.pulse_re[k] * q * q * p.corr), sim = (float)(p.pulse_im[k] * q * q * p.corr);
+#pragma unroll
+      for (int i = 0; i < MAXT; ++i) {
+        const int t = t0 + 2 * i + tg;
+        if (t < p.NT) out[((size_t)t * p.NF + k) * p.E + e] = cmul(acc[i], cf{sre, sim});
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- host side
+struct MsmPlan {
+  int ET, FBK, nfb, nfpad, chunks, s_per_chunk;
+  size_t lds, kpart_bytes;
+};
+
+static MsmPlan msm_plan(const Problem& p) {
+  MsmPlan m{};
+  m.ET = p.E <= 64 ? 4 : 8;
+  m.FBK = 2;
+  m.nfb = (p.NF + m.FBK - 1) / m.FBK;
+  m.nfpad = m.nfb * m.FBK;
+  const int tiles = (p.S + MST - 1) / MST;
+  int chunks = std::max(1, (2048 + m.nfb - 1) / m.nfb);
+  chunks = std::min(chunks, std::max(1, tiles / 8));  // keep >= 8 tiles per chunk
+  m.s_per_chunk = ((tiles + chunks - 1) / chunks) * MST;
+  m.chunks = (p.S + m.s_per_chunk - 1) / m.s_per_chunk;
+  if (m.chunks < 1) m.chunks = 1;
+  const size_t ep = (size_t)m.ET * 16;
+  m.kpart_bytes = (size_t)m.chunks * m.nfpad * ep * ep * sizeof(cf);
+  m.lds = (size_t)m.FBK * 2 * 2 * m.ET * 64 * 16;
+  return m;
+}
+
+size_t mfma_workspace(const Problem& p) {
+  if (p.E > 128 || p.NS > 4) return 256;
+  return (msm_plan(p).kpart_bytes + 255) / 256 * 256;
+}
+
+template <int NSUB, int ET, int FBK, bool FD>
+static int launch_msm(const Problem& p, const MsmPlan& m, cf* kpart, hipStream_t st) {
+  auto kern = k_msm<NSUB, ET, FBK, FD>;
+  if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds) != hipSuccess)
+    return FS_ERR_RESOURCE;
+  const int slot = timing_start(st);
+  hipLaunchKernelGGL(kern, dim3(m.nfb, m.chunks), dim3(256), m.lds, st, p, m.s_per_chunk, m.nfpad, kpart);
+  timing_stop(slot, st);
+  return FS_OK;
+}
+
+template <int NSUB, int ET>
+static int launch_msm_fd(const Problem& p, const MsmPlan& m, cf* kpart, hipStream_t st) {
+  return p.fulldir ? launch_msm<NSUB, ET, 2, true>(p, m, kpart, st) : launch_msm<NSUB, ET, 2, false>(p, m, kpart, st);
+}
+
+template <int NSUB>
+static int launch_msm_et(const Problem& p, const MsmPlan& m, cf* kpart, hipStream_t st) {
+  return m.ET == 4 ? launch_msm_fd<NSUB, 4>(p, m, kpart, st) : launch_msm_fd<NSUB, 8>(p, m, kpart, st);
+}
+
+int mfma_spectrum(const Problem& p, cf* spect, char* ws, size_t ws_bytes, hipStream_t st) {
+  if (p.E > 128) return FS_ERR_UNSUPPORTED;
+  const MsmPlan m = msm_plan(p);
+  if (ws_bytes < m.kpart_bytes) return FS_ERR_WORKSPACE;
+  cf* kpart = (cf*)ws;
+  int rc;
+  switch (p.NS) {
+    case 1: rc = launch_msm_et<1>(p, m, kpart, st); break;
+    case 2: rc = launch_msm_et<2>(p, m, kpart, st); break;
+    case 3: rc = launch_msm_et<3>(p, m, kpart, st); break;
+    case 4: rc = launch_msm_et<4>(p, m, kpart, st); break;
+    default: return FS_ERR_UNSUPPORTED;
+  }
+  if (rc != FS_OK) return rc;
+  hipLaunchKernelGGL(k_msm_apply, dim3(p.NF), dim3(256), 0, st, p, m.ET * 16, m.chunks, m.nfpad, kpart, spect);
+  return hipGetLastError() == hipSuccess ? FS_OK : FS_ERR_LAUNCH;
+}
+
+}  // namespace fs

@@ -2,8 +2,6 @@
 #include "fs_device.h"
 
 namespace fs {
-int mfma_spectrum(const Problem&, cf*, char*, size_t, hipStream_t) { return FS_ERR_UNSUPPORTED; }
-size_t mfma_workspace(const Problem&) { return 256; }
 int adjoint_fp32(const Problem&, const cf*, float*, float*, float*, double*, char*, hipStream_t) {
   return FS_ERR_UNSUPPORTED;
 }

@@ -129,6 +129,14 @@ int fs_simus_adjoint(const fs_simus_desc* d, const float* d_spect, float* d_rc, 
 size_t fs_pfield_workspace_bytes(const fs_simus_desc* d);
 int fs_pfield(const fs_simus_desc* d, float* out, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Optional timing of the dominant kernel launches (k_msm for FS_METHOD_MFMA_BF16X3, k_tx + k_rx for
+ * FS_METHOD_FP32): while enabled, each such launch is bracketed by HIP events on its stream.
+ * fs_kernel_time_ms waits for the recorded launches, returns their summed duration in milliseconds,
+ * stores the launch count in *launches (if not NULL) and clears the record. Process-wide and not
+ * thread-safe (meant for benchmarks). No counterpart in the reference. */
+int fs_set_kernel_timing(int enable);
+double fs_kernel_time_ms(int* launches);
+
 #ifdef __cplusplus
 }
 #endif

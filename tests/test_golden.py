@@ -1,0 +1,84 @@
+"""Golden fixtures (tests/golden/*.npz, written by make_golden.py from the FP64 oracle).
+
+CPU: the product's host plan reproduces every fixture's plan scalars, and the
+oracle reproduces its own stored outputs (regression pin of the checker).
+GPU (marked ``gpu``): the HIP path reproduces the stored spectrum and RF.
+"""
+
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import fastsimus_amd as fs
+from fastsimus_amd.plan import simus_precompute
+from oracle import simus_oracle as O
+from tests.conftest import rel_l2
+
+GOLDEN = sorted((Path(__file__).resolve().parent / "golden").glob("*.npz"))
+RF_TOL = 1e-4  # plain FP32 path vs FP64 oracle, RF relative L2
+
+
+def _load(path):
+    g = np.load(path, allow_pickle=False)
+    # JSON writes +inf (linear radius, unfocused elevation) as null: drop those keys to get the defaults back
+    params = fs.TransducerParams(**{k: v for k, v in json.loads(str(g["params"])).items() if v is not None})
+    medium = fs.MediumParams(**json.loads(str(g["medium"])))
+    return g, params, medium
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=lambda p: p.stem)
+def test_host_plan_matches_fixture(path):
+    g, params, medium = _load(path)
+    sc = np.stack([g["x"], g["z"]], -1)
+    plan = simus_precompute(sc, g["rc"], g["delays"], params, medium)
+    assert plan.selected_freqs.shape[0] == int(g["n_f"])
+    assert (plan.n_freq_full, plan.freq_idx_start, plan.n_fft, plan.n_sub) == (
+        int(g["n_ff"]), int(g["idx0"]), int(g["n_fft"]), int(g["n_sub"]))
+    assert plan.correction_factor == pytest.approx(float(g["corr"]), rel=1e-14)
+
+
+@pytest.mark.parametrize("path", [p for p in GOLDEN if p.stem in ("p4_focused", "brute_tiny", "convex_c5")],
+                         ids=lambda p: p.stem)
+def test_oracle_reproduces_fixture(path):
+    g, params, medium = _load(path)
+    plan = O.simus_plan(g["x"], g["z"], g["delays"], params, medium)
+    chan = g["chan"]
+    for t, d in enumerate(g["delays"]):
+        rf, _, sel = O.simus_oracle(g["x"], g["z"], g["rc"], d, params, medium, plan)
+        assert rel_l2(sel[:, chan], g["spect"][t]) < 1e-6
+        assert rel_l2(rf[:, chan], g["rf"][t]) < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", GOLDEN, ids=lambda p: p.stem)
+def test_hip_fp32_matches_fixture(path, cuda):
+    g, params, medium = _load(path)
+    sc = np.stack([g["x"], g["z"]], -1)
+    delays = g["delays"] if g["delays"].shape[0] > 1 else g["delays"][0]
+    res = fs.simus(sc, g["rc"], delays, params, medium)
+    rf = res.rf.cpu().numpy().reshape(-1, *res.rf.shape[-2:])
+    chan = g["chan"]
+    print(path.stem, flush=True)
+    for t in range(g["rf"].shape[0]):
+        assert rel_l2(rf[t][:, chan], g["rf"][t]) < RF_TOL
+        full = res.spectrum.cpu().numpy().reshape(-1, *res.spectrum.shape[-2:])[t]
+        i0 = int(g["idx0"])
+        assert rel_l2(full[i0:i0 + int(g["n_f"])][:, chan], g["spect"][t]) < RF_TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", [p for p in GOLDEN if p.stem in ("two_tx", "cfg0_sub64", "convex_c5")],
+                         ids=lambda p: p.stem)
+def test_hip_mfma_matches_fixture(path, cuda):
+    g, params, medium = _load(path)
+    sc = np.stack([g["x"], g["z"]], -1)
+    res = fs.simus(sc, g["rc"], g["delays"], params, medium, strategy="hip_mfma")
+    rf = res.rf.cpu().numpy()
+    chan = g["chan"]
+    print(path.stem, flush=True)
+    for t in range(g["rf"].shape[0]):
+        assert rel_l2(rf[t][:, chan], g["rf"][t]) < 1e-3

@@ -1,0 +1,223 @@
+"""Host-side logic of the product (CPU only): parameter models, geometry, spectra,
+delay laws and frequency plans, checked against the oracle's independent
+restatement and the reference's known answers; plus the C-ABI library's
+exported symbols and argument checks (no kernel launches)."""
+
+from __future__ import annotations
+
+import ctypes as C
+import re
+from math import inf, pi
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import fastsimus_amd as fs
+from fastsimus_amd import _lib
+from fastsimus_amd.plan import pfield_precompute, simus_precompute, two_way_pulse_duration
+from oracle import simus_oracle as O
+from tests.conftest import ROOT, speckle
+
+
+# ----------------------------------------------------------------- parameter models (transducer_params.py:135-160)
+def test_params_width_or_kerf():
+    with pytest.raises(ValueError):
+        fs.TransducerParams(freq_center=1e6, pitch=3e-4, n_elements=8)
+    with pytest.raises(ValueError):
+        fs.TransducerParams(freq_center=1e6, pitch=3e-4, n_elements=8, width=2e-4, kerf=1e-4)
+    with pytest.raises(ValueError):
+        fs.TransducerParams(freq_center=1e6, pitch=3e-4, n_elements=8, width=4e-4)
+    with pytest.raises(ValueError):
+        fs.TransducerParams(freq_center=1e6, pitch=3e-4, n_elements=8, kerf=3e-4)
+    p = fs.TransducerParams(freq_center=2.5e6, pitch=300e-6, n_elements=128, width=250e-6)
+    assert p.element_width == 250e-6 and p.kerf_width == pytest.approx(5e-5)
+    q = fs.TransducerParams(freq_center=3.5e6, pitch=400e-6, n_elements=64, kerf=50e-6, radius=50e-3)
+    assert q.element_width == pytest.approx(350e-6)
+
+
+def test_params_frozen_and_baffle():
+    p = fs.L11_5v()
+    with pytest.raises(Exception):
+        p.pitch = 1.0
+    assert fs.TransducerParams(freq_center=1e6, pitch=3e-4, n_elements=2, width=1e-4, baffle="rigid").baffle == fs.BaffleType.RIGID
+    assert fs.TransducerParams(freq_center=1e6, pitch=3e-4, n_elements=2, width=1e-4, baffle=1.75).baffle == 1.75
+    with pytest.raises(ValueError):
+        fs.MediumParams(speed_of_sound=0)
+    with pytest.raises(ValueError):
+        fs.MediumParams(attenuation=-1)
+
+
+def test_presets():
+    assert fs.P4_2v().n_elements == 64 and fs.P4_2v().element_width == pytest.approx(250e-6)
+    assert fs.L11_5v().element_width == pytest.approx(270e-6)
+    assert fs.L12_3v().n_elements == 192
+    assert fs.C5_2v().radius == pytest.approx(49.57e-3)
+
+
+# ----------------------------------------------------------------- geometry / spectra / delays vs oracle
+@pytest.mark.parametrize("preset", [fs.P4_2v, fs.L11_5v, fs.C5_2v])
+def test_element_positions_match_oracle(preset):
+    p = preset()
+    pos, theta, apex = fs.element_positions(p.n_elements, p.pitch, p.radius)
+    ex, ez, th, ap = O.element_layout(p)
+    np.testing.assert_array_equal(pos[:, 0], ex)
+    np.testing.assert_allclose(pos[:, 1], ez, atol=1e-15)
+    assert apex == pytest.approx(ap)
+    if p.radius == inf:
+        assert theta is None
+    else:
+        np.testing.assert_allclose(theta, th)
+        # convex chord endpoints sit at z = 0 (geometry.py:70-84)
+        assert abs(pos[0, 1]) < 1e-12 and abs(pos[-1, 1]) < 1e-12
+
+
+def test_spectra_match_oracle():
+    w = 2 * pi * np.linspace(0, 15e6, 333)
+    np.testing.assert_allclose(fs.spectrum.pulse_spectrum(w, 7.6e6, 2), O.pulse_spectrum(w, 7.6e6, 2), atol=1e-15)
+    np.testing.assert_allclose(fs.spectrum.probe_spectrum(w, 7.6e6, 0.77), O.probe_spectrum(w, 7.6e6, 0.77), rtol=1e-12)
+    with pytest.raises(ValueError):
+        fs.spectrum.probe_spectrum(w, 7.6e6, 0.0)
+
+
+def test_plane_wave_and_focused_delays():
+    p = fs.L11_5v()
+    pos, _, _ = fs.element_positions(128, p.pitch, p.radius)
+    d0 = fs.plane_wave(pos, 0.0, speed_of_sound=1540.0)
+    assert np.all(d0 == 0)  # tests/test_tx_delay.py:120-133
+    d = fs.plane_wave(pos, np.deg2rad(10), speed_of_sound=1540.0)
+    assert d.min() == 0 and np.all(np.diff(d) > 0)
+    np.testing.assert_allclose(np.diff(d), p.pitch * np.sin(np.deg2rad(10)) / 1540.0, rtol=1e-5)
+    with pytest.raises(ValueError):
+        fs.plane_wave(pos, pi / 2, speed_of_sound=1540.0)
+    f = fs.focused(pos, np.array([0.0, 0.03]), speed_of_sound=1540.0)
+    # converging wave: edge elements fire first (zero delay), centre last
+    assert f[0] == pytest.approx(0.0) and np.argmax(f) in (63, 64)
+    dv = fs.focused(pos, np.array([0.0, -0.01]), speed_of_sound=1540.0)
+    assert np.argmin(dv) in (63, 64)
+    dw = fs.diverging_wave(pos, 0.0, np.deg2rad(60), aperture_length=127 * p.pitch, speed_of_sound=1540.0)
+    assert dw.min() == 0 and np.argmin(dw) in (63, 64)
+
+
+def test_delays_keep_torch_type():
+    torch = pytest.importorskip("torch")
+    p = fs.L11_5v()
+    pos, _, _ = fs.element_positions(128, p.pitch, p.radius)
+    d = fs.plane_wave(torch.as_tensor(pos), 0.1, speed_of_sound=1540.0)
+    assert isinstance(d, torch.Tensor) and d.shape == (128,)
+
+
+# ----------------------------------------------------------------- plans vs oracle
+@pytest.mark.parametrize("preset,seed", [(fs.L11_5v, 0), (fs.P4_2v, 1), (fs.C5_2v, 2)])
+def test_simus_plan_matches_oracle(preset, seed):
+    p = preset()
+    x, z, rc = speckle(seed, 40, zlim=(0.005, 0.06))
+    pos, _, _ = fs.element_positions(p.n_elements, p.pitch, p.radius)
+    d = fs.plane_wave(pos, 0.05, speed_of_sound=1540.0) if p.radius == inf else np.zeros(p.n_elements)
+    m = fs.MediumParams(attenuation=0.3)
+    plan = simus_precompute(np.stack([x, z], -1), rc, d, p, m)
+    o = O.simus_plan(x, z, d, p, m)
+    np.testing.assert_array_equal(plan.selected_freqs, o["freqs"])
+    np.testing.assert_allclose(plan.pulse_spectrum, o["pulse"], atol=1e-15)
+    np.testing.assert_allclose(plan.probe_spectrum, o["probe"], rtol=1e-12)
+    assert (plan.n_sub, plan.n_freq_full, plan.freq_idx_start, plan.n_fft) == (o["n_sub"], o["n_ff"], o["idx0"], o["n_fft"])
+    assert plan.correction_factor == pytest.approx(o["corr"], rel=1e-15)
+    assert plan.seg_length == pytest.approx(o["seg"])
+
+
+def test_two_way_duration_matches_oracle():
+    for fc, bw, n in [(2.72e6, 0.74, 1), (7.6e6, 0.77, 1), (5e6, 0.6, 2.5)]:
+        assert two_way_pulse_duration(fc, bw, n) == pytest.approx(O.two_way_duration(fc, bw, n), abs=1e-15)
+
+
+def test_plan_known_answer_p4_focused():
+    """tests/test_simus.py:355-386."""
+    p = fs.P4_2v()
+    sc = np.stack([np.zeros(6), np.linspace(1e-2, 8e-2, 6)], -1)
+    pos, _, _ = fs.element_positions(64, p.pitch, p.radius)
+    d = fs.focused(pos, np.array([0.0, 0.03]), speed_of_sound=1540.0)
+    plan = simus_precompute(sc, np.ones(6), d, p, fs=4 * p.freq_center)
+    assert abs(plan.selected_freqs.shape[0] - 812) < 20
+
+
+def test_batched_plan_uses_largest_delay():
+    p = fs.L11_5v()
+    pos, _, _ = fs.element_positions(128, p.pitch, p.radius)
+    D = np.stack([fs.plane_wave(pos, np.deg2rad(a), speed_of_sound=1540.0) for a in (-16, 0, 10)])
+    x, z, rc = speckle(3, 30)
+    sc = np.stack([x, z], -1)
+    plan_b = simus_precompute(sc, rc, D, p)
+    plan_0 = simus_precompute(sc, rc, D[0], p)  # the -16 deg firing has the largest span
+    np.testing.assert_array_equal(plan_b.selected_freqs, plan_0.selected_freqs)
+
+
+def test_nan_delays_ignored_in_plan():
+    p = fs.L11_5v()
+    x, z, rc = speckle(4, 10)
+    d = np.zeros(128)
+    d[3] = np.nan
+    a = simus_precompute(np.stack([x, z], -1), rc, d, p)
+    b = simus_precompute(np.stack([x, z], -1), rc, np.zeros(128), p)
+    np.testing.assert_array_equal(a.selected_freqs, b.selected_freqs)
+
+
+def test_pfield_plan_matches_oracle():
+    p = fs.L11_5v()
+    x, z, _ = speckle(5, 30)
+    d = np.zeros(128)
+    plan = pfield_precompute(np.stack([x, z], -1), d, p)
+    o = O.pfield_plan(x, z, d, p, fs.MediumParams())
+    np.testing.assert_array_equal(plan.selected_freqs, o["freqs"])
+    assert plan.correction_factor == pytest.approx(o["corr"], rel=1e-15)
+    with pytest.raises(ValueError):
+        pfield_precompute(np.zeros((0, 2)), d, p)
+
+
+def test_strategy_names():
+    from fastsimus_amd.simus import _method
+
+    assert _method(None) == "fp32" and _method("cuda") == "fp32" and _method("hip_mfma") == "mfma"
+    for s in ("python", "scan", "metal"):
+        with pytest.raises(NotImplementedError):
+            _method(s)
+
+
+# ----------------------------------------------------------------- the C-ABI library (loads, exports, rejects; no GPU needed)
+def _declared_symbols():
+    hdr = (ROOT / "include" / "fastsimus.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|double|const char\*)\s+(fs_\w+)\s*\(", hdr, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    declared = _declared_symbols()
+    assert set(declared) == set(_lib.EXPORTS)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.fs_version() >= 100
+    assert lib.fs_status_string(0) == b"ok"
+
+
+def test_library_rejects_bad_descriptors():
+    lib = _lib.load()
+    d = _lib.SimusDesc()
+    assert lib.fs_simus_workspace_bytes(C.byref(d), 0) == 0  # empty descriptor
+    assert lib.fs_simus_spectrum(None, 0, None, None, 0, None) == _lib.FS_ERR_INVALID
+    assert lib.fs_spectrum_to_rf(None, 1, 1, 1, 1, 0, 4, None, None, None, None, 0, None) == _lib.FS_ERR_INVALID
+    assert lib.fs_pfield(None, None, None, 0, None) == _lib.FS_ERR_INVALID
+    assert lib.fs_simus_adjoint(None, None, None, None, None, None, None, 0, None) == _lib.FS_ERR_INVALID
+    with pytest.raises(ValueError):
+        _lib.check(_lib.FS_ERR_INVALID, "x")
+    with pytest.raises(NotImplementedError):
+        _lib.check(_lib.FS_ERR_UNSUPPORTED, "x")
+    with pytest.raises(RuntimeError):
+        _lib.check(_lib.FS_ERR_WORKSPACE, "x")
+
+
+def test_compute_fails_loudly_without_gpu():
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    p = fs.L11_5v()
+    with pytest.raises(RuntimeError, match="GPU"):
+        fs.simus(np.array([[0.0, 0.02]]), np.ones(1), np.zeros(128), p)
