@@ -25,6 +25,7 @@ EXPORTS = (
     "fs_simus_spectrum",
     "fs_rf_workspace_bytes",
     "fs_spectrum_to_rf",
+    "fs_rf_adjoint_workspace_bytes",
     "fs_rf_adjoint",
     "fs_simus_adjoint_workspace_bytes",
     "fs_simus_adjoint",
@@ -77,6 +78,8 @@ def load() -> C.CDLL:
     lib.fs_spectrum_to_rf.restype = C.c_int
     lib.fs_spectrum_to_rf.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P, P,
                                       C.c_size_t, P]
+    lib.fs_rf_adjoint_workspace_bytes.restype = C.c_size_t
+    lib.fs_rf_adjoint_workspace_bytes.argtypes = [C.c_int, C.c_int, C.c_int]
     lib.fs_rf_adjoint.restype = C.c_int
     lib.fs_rf_adjoint.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, C.c_size_t, P]
     lib.fs_simus_adjoint_workspace_bytes.restype = C.c_size_t

@@ -130,6 +130,11 @@ int fs_spectrum_to_rf(const float* spect, int n_tx, int n_freq, int n_elem, int 
                         (unsigned*)workspace, (hipStream_t)stream);
 }
 
+size_t fs_rf_adjoint_workspace_bytes(int n_tx, int n_keep, int n_elem) {
+  if (n_tx <= 0 || n_keep <= 0 || n_elem <= 0) return 0;
+  return rf_adjoint_ws(n_tx, n_keep, n_elem);
+}
+
 int fs_rf_adjoint(const float* rf_ungated, const float* d_rf, int n_tx, int n_freq, int n_elem, int idx0, int n_fft,
                   float* d_spect, void* workspace, size_t workspace_bytes, void* stream) {
   if (!rf_ungated || !d_rf || !d_spect || n_tx <= 0 || n_freq <= 0 || n_elem <= 0 || n_fft <= 0 || idx0 < 0)

@@ -44,6 +44,11 @@ __global__ void k_tx_table(Problem p, int nfp, cf* __restrict__ A) {
   }
 }
 
+void launch_tx_table(const Problem& p, int nfp, cf* A, hipStream_t st) {
+  const long n = (long)p.NT * p.E * nfp;
+  hipLaunchKernelGGL(k_tx_table, dim3((int)std::min<long>((n + 255) / 256, 65536)), dim3(256), 0, st, p, nfp, A);
+}
+
 // PP(f_k) = pulse * probe, and probe, padded with zeros beyond NF.
 __global__ void k_spectra(Problem p, int nfp, cf* __restrict__ pp, float* __restrict__ pr) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;

@@ -390,9 +390,8 @@ static MsmPlan msm_plan(const Problem& p) {
   const int tiles = (p.S + MST - 1) / MST;
   int chunks = std::max(1, (2048 + m.nfb - 1) / m.nfb);
   chunks = std::min(chunks, std::max(1, tiles / 8));  // keep >= 8 tiles per chunk
-  m.s_per_chunk = ((tiles + chunks - 1) / chunks) * MST;
-  m.chunks = (p.S + m.s_per_chunk - 1) / m.s_per_chunk;
-  if (m.chunks < 1) m.chunks = 1;
+  m.s_per_chunk = std::max(1, (tiles + chunks - 1) / chunks) * MST;
+  m.chunks = std::max(1, (p.S + m.s_per_chunk - 1) / m.s_per_chunk);
   const size_t ep = (size_t)m.ET * 16;
   m.kpart_bytes = (size_t)m.chunks * m.nfpad * ep * ep * sizeof(cf);
   m.lds = (size_t)m.FBK * 2 * 2 * m.ET * 64 * 16;

@@ -114,6 +114,7 @@ int fs_spectrum_to_rf(const float* spect, int n_tx, int n_freq, int n_elem, int 
 /* Reverse of fs_spectrum_to_rf: given dL/d(rf) of the gated RF and the pre-gate RF, writes
  * dL/d(spect) for the selected band (interleaved complex float32, [n_tx][n_freq][n_elem]);
  * the complex cotangent convention is g = dL/dRe + i dL/dIm. */
+size_t fs_rf_adjoint_workspace_bytes(int n_tx, int n_keep, int n_elem);
 int fs_rf_adjoint(const float* rf_ungated, const float* d_rf, int n_tx, int n_freq, int n_elem, int idx0, int n_fft,
                   float* d_spect, void* workspace, size_t workspace_bytes, void* stream);
 

@@ -1,0 +1,241 @@
+"""GPU parity of the HIP SIMUS / PFIELD kernels against the FP64 oracle.
+
+Tolerances (RF relative L2 vs FP64): plain FP32 method 1e-4, split-BF16 MFMA
+method 1e-3. Cases follow the reference's tests: its two PyMUST geometries
+(tests/test_simus.py:68-97), edge cases (single scatterer, zero rc, attenuation,
+NaN delays, out-of-field scatterers), convex arrays, baffles, sub-element
+counts, full-frequency directivity, several transmits. At full workload size
+the checks are size-independent properties (linearity, mirror symmetry,
+MFMA vs FP32 agreement, run-to-run reproducibility).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import fastsimus_amd as fs
+from oracle import simus_oracle as O
+from tests.conftest import f32, rel_l2, speckle
+
+pytestmark = pytest.mark.gpu
+FP32_TOL = 1e-4
+MFMA_TOL = 1e-3
+
+
+def _oracle_rf(x, z, rc, D, params, medium, **kw):
+    D = np.atleast_2d(D)
+    plan = O.simus_plan(x, z, D, params, medium, element_splitting=kw.get("element_splitting"))
+    fd = kw.get("full_frequency_directivity", False)
+    return np.stack([O.simus_oracle(x, z, rc, d, params, medium, plan, full_frequency_directivity=fd)[0] for d in D])
+
+
+def _run(strategy, x, z, rc, D, params, medium, **kw):
+    res = fs.simus(np.stack([x, z], -1), rc, D, params, medium, strategy=strategy, **kw)
+    rf = res.rf.cpu().numpy()
+    return rf.reshape(-1, *rf.shape[-2:]), res
+
+
+def _pos(p):
+    return fs.element_positions(p.n_elements, p.pitch, p.radius)[0]
+
+
+CASES = {
+    "p4_focused_ref": lambda: (fs.P4_2v(), fs.MediumParams(), f32(np.zeros(6)), f32(np.linspace(1e-2, 8e-2, 6)),
+                               f32(np.ones(6)), fs.focused(_pos(fs.P4_2v()), np.array([0.0, 0.03]), speed_of_sound=1540.0), {}),
+    "l11_plane5_ref": lambda: (fs.L11_5v(), fs.MediumParams(), f32(np.linspace(-1e-2, 1e-2, 6)),
+                               f32(np.linspace(0.5e-2, 3e-2, 6)), f32(np.ones(6)),
+                               fs.plane_wave(_pos(fs.L11_5v()), np.deg2rad(5), speed_of_sound=1540.0), {}),
+    "l11_speckle_atten": lambda: (fs.L11_5v(), fs.MediumParams(attenuation=0.5), *speckle(1, 80), np.zeros(128), {}),
+    "rigid_nsub1": lambda: (fs.L11_5v().model_copy(update={"baffle": "rigid"}), fs.MediumParams(), *speckle(2, 30),
+                            np.zeros(128), {"element_splitting": 1}),
+    "impedance_nsub3": lambda: (fs.L11_5v().model_copy(update={"baffle": 1.75}), fs.MediumParams(attenuation=0.3),
+                                *speckle(3, 30), np.zeros(128), {"element_splitting": 3}),
+    "convex_c5": lambda: (fs.C5_2v(), fs.MediumParams(attenuation=0.4), *speckle(4, 40, xlim=(-0.03, 0.03), zlim=(0.005, 0.06)),
+                          np.zeros(128), {}),
+    "full_freq_directivity": lambda: (fs.L11_5v(), fs.MediumParams(), *speckle(5, 20), np.zeros(128),
+                                      {"full_frequency_directivity": True}),
+    "nan_delays": lambda: (fs.L11_5v(), fs.MediumParams(), *speckle(6, 20),
+                           np.where(np.arange(128) % 17 == 3, np.nan, 0.0), {}),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fp32_matches_oracle(name, cuda):
+    params, medium, x, z, rc, D, kw = CASES[name]()
+    print(name, flush=True)
+    rf, _ = _run("hip", x, z, rc, D, params, medium, **kw)
+    want = _oracle_rf(x, z, rc, D, params, medium, **kw)
+    assert rel_l2(rf, want) < FP32_TOL
+
+
+@pytest.mark.parametrize("name", ["l11_speckle_atten", "convex_c5", "impedance_nsub3", "p4_focused_ref",
+                                  "full_freq_directivity"])
+def test_mfma_matches_oracle(name, cuda):
+    params, medium, x, z, rc, D, kw = CASES[name]()
+    print(name, flush=True)
+    rf, _ = _run("hip_mfma", x, z, rc, D, params, medium, **kw)
+    want = _oracle_rf(x, z, rc, D, params, medium, **kw)
+    assert rel_l2(rf, want) < MFMA_TOL
+
+
+def test_multi_transmit_both_methods(cuda):
+    p = fs.L11_5v()
+    m = fs.MediumParams(attenuation=0.5)
+    x, z, rc = speckle(7, 50)
+    D = np.stack([fs.plane_wave(_pos(p), np.deg2rad(a), speed_of_sound=1540.0) for a in (-12, -3, 0, 7, 15)])
+    want = _oracle_rf(x, z, rc, D, p, m)
+    rf32, res = _run("hip", x, z, rc, D, p, m)
+    rfmm, _ = _run("hip_mfma", x, z, rc, D, p, m)
+    assert res.rf.shape[0] == 5 and res.spectrum.shape[0] == 5
+    assert rel_l2(rf32, want) < FP32_TOL
+    assert rel_l2(rfmm, want) < MFMA_TOL
+
+
+def test_phased_array_focused_beams_mfma(cuda):
+    p = fs.P4_2v()
+    x, z, rc = speckle(8, 60, xlim=(-0.04, 0.04), zlim=(0.01, 0.1))
+    ang = np.deg2rad(np.linspace(-40, 40, 9))
+    D = np.stack([fs.focused(_pos(p), np.array([0.06 * np.sin(a), 0.06 * np.cos(a)]), speed_of_sound=1540.0) for a in ang])
+    want = _oracle_rf(x, z, rc, D, p, fs.MediumParams())
+    rf, _ = _run("hip_mfma", x, z, rc, D, p, fs.MediumParams())
+    assert rel_l2(rf, want) < MFMA_TOL
+
+
+# ----------------------------------------------------------------- edge cases of the reference tests
+def test_single_scatterer_and_zero_rc(cuda):
+    """tests/test_simus.py:390-411."""
+    p = fs.P4_2v()
+    rf, _ = _run("hip", f32([0.0]), f32([3e-2]), f32([1.0]), np.zeros(64), p, fs.MediumParams())
+    assert rf.shape[-1] == 64 and np.abs(rf).max() > 0
+    x, z = f32(np.zeros(3)), f32(np.linspace(1e-2, 5e-2, 3))
+    for strat in ("hip", "hip_mfma"):
+        rf, _ = _run(strat, x, z, f32(np.zeros(3)), np.zeros(64), p, fs.MediumParams())
+        assert np.abs(rf).max() < 1e-10
+
+
+def test_attenuation_lowers_power(cuda):
+    """tests/test_simus.py:413-427."""
+    p = fs.P4_2v()
+    x, z, rc = f32(np.zeros(3)), f32(np.linspace(1e-2, 5e-2, 3)), f32(np.ones(3))
+    rf0, _ = _run("hip", x, z, rc, np.zeros(64), p, fs.MediumParams(attenuation=0.0))
+    rf1, _ = _run("hip", x, z, rc, np.zeros(64), p, fs.MediumParams(attenuation=0.5))
+    assert np.sum(rf1**2) < np.sum(rf0**2)
+
+
+def test_out_of_field_and_empty(cuda):
+    p = fs.L11_5v()
+    x, z, rc = speckle(9, 20)
+    a, _ = _run("hip", x, z, rc, np.zeros(128), p, fs.MediumParams())
+    b, _ = _run("hip", np.append(x, 0.0), np.append(z, -0.01), np.append(rc, 3.0), np.zeros(128), p, fs.MediumParams())
+    assert rel_l2(b, a) < 1e-6
+    plan = fs.simus_precompute(np.stack([x, z], -1), rc, np.zeros(128), p)
+    for strat in ("hip", "hip_mfma"):
+        r = fs.simus_compute(np.zeros((0, 2)), np.zeros(0), np.zeros(128), plan, p, strategy=strat)
+        assert r.rf.shape == (plan.n_fft // 2, 128) and float(r.rf.abs().max()) == 0.0
+
+
+def test_scatterer_on_element_is_clipped(cuda):
+    p = fs.L11_5v()
+    x = f32([float(_pos(p)[40, 0]), 0.0])
+    z = f32([0.0, 0.02])
+    rc = f32([1.0, 1.0])
+    rf, _ = _run("hip", x, z, rc, np.zeros(128), p, fs.MediumParams())
+    want = _oracle_rf(x, z, rc, np.zeros(128), p, fs.MediumParams())
+    assert np.all(np.isfinite(rf)) and rel_l2(rf, want) < FP32_TOL
+
+
+def test_precompute_compute_equals_simus_and_reproducible(cuda):
+    """tests/test_simus.py:302-319, plus bitwise run-to-run reproducibility (no atomics in the sums)."""
+    import torch
+
+    p = fs.P4_2v()
+    sc = np.stack([np.zeros(3), np.linspace(1e-2, 5e-2, 3)], -1)
+    a = fs.simus(sc, np.ones(3), np.zeros(64), p)
+    plan = fs.simus_precompute(sc, np.ones(3), np.zeros(64), p)
+    b = fs.simus_compute(sc, np.ones(3), np.zeros(64), plan, p)
+    assert torch.equal(a.rf, b.rf) and torch.equal(a.spectrum, b.spectrum)
+    x, z, rc = speckle(10, 3000)
+    for strat in ("hip", "hip_mfma"):
+        r1 = fs.simus(np.stack([x, z], -1), rc, np.zeros(64), p, strategy=strat)
+        r2 = fs.simus(np.stack([x, z], -1), rc, np.zeros(64), p, strategy=strat)
+        assert torch.equal(r1.rf, r2.rf)
+
+
+def test_torch_inputs_match_numpy(cuda):
+    import torch
+
+    p = fs.L11_5v()
+    x, z, rc = speckle(11, 40)
+    sc = np.stack([x, z], -1)
+    a = fs.simus(sc, rc, np.zeros(128), p)
+    b = fs.simus(torch.as_tensor(sc, device=cuda), torch.as_tensor(rc, device=cuda),
+                 torch.zeros(128, dtype=torch.float64, device=cuda), p)
+    assert torch.equal(a.rf, b.rf)
+    assert b.rf.device.type == "cuda"
+
+
+def test_spectrum_layout_and_rf_tail(cuda):
+    p = fs.L11_5v()
+    m = fs.MediumParams()
+    x, z, rc = speckle(12, 10)
+    plan_o = O.simus_plan(x, z, np.zeros(128), p, m)
+    rf_o, full_o, _ = O.simus_oracle(x, z, rc, np.zeros(128), p, m, plan_o)
+    _, res = _run("hip", x, z, rc, np.zeros(128), p, m)
+    full = res.spectrum.cpu().numpy()
+    assert full.shape == full_o.shape
+    assert np.abs(full[: plan_o["idx0"]]).max() == 0.0
+    assert rel_l2(full, full_o) < FP32_TOL
+
+
+# ----------------------------------------------------------------- full-size, size-independent properties
+def _cfg1():
+    rng = np.random.default_rng(2026)
+    p = fs.TransducerParams(freq_center=7.6e6, pitch=3.0e-4, width=2.7e-4, n_elements=128, bandwidth=0.77)
+    x = np.concatenate([rng.uniform(-0.02, 0.02, 50000), np.zeros(10)])
+    z = np.concatenate([rng.uniform(0.005, 0.05, 50000), np.arange(1, 11) * 5e-3])
+    rc = np.concatenate([rng.standard_normal(50000), np.full(10, 10.0)])
+    return p, fs.MediumParams(attenuation=0.5), f32(x), f32(z), f32(rc)
+
+
+def test_full_size_linearity_and_symmetry(cuda):
+    import torch
+
+    p, m, x, z, rc = _cfg1()
+    sc = np.stack([x, z], -1)
+    plan = fs.simus_precompute(sc, rc, np.zeros(128), p, m)
+    half = rc.copy()
+    half[::2] = 0.0
+    rest = rc - half
+    s_all = fs.simus_compute(sc, rc, np.zeros(128), plan, p, m).spectrum
+    s_a = fs.simus_compute(sc, half, np.zeros(128), plan, p, m).spectrum
+    s_b = fs.simus_compute(sc, rest, np.zeros(128), plan, p, m).spectrum
+    assert float(torch.linalg.norm(s_a + s_b - s_all) / torch.linalg.norm(s_all)) < 1e-5
+    mir = fs.simus_compute(np.stack([-x, z], -1), rc, np.zeros(128), plan, p, m).spectrum
+    assert float(torch.linalg.norm(mir.flip(-1) - s_all) / torch.linalg.norm(s_all)) < 1e-5
+    mm = fs.simus_compute(sc, rc, np.zeros(128), plan, p, m, strategy="hip_mfma").rf
+    r32 = fs.simus_compute(sc, rc, np.zeros(128), plan, p, m).rf
+    assert float(torch.linalg.norm(mm - r32) / torch.linalg.norm(r32)) < MFMA_TOL
+
+
+# ----------------------------------------------------------------- pfield
+def test_pfield_matches_oracle(cuda):
+    p = fs.L11_5v()
+    g = f32(np.stack(np.meshgrid(np.linspace(-0.012, 0.012, 13), np.linspace(0.004, 0.035, 11)), -1))
+    d = fs.focused(_pos(p), np.array([0.003, 0.02]), speed_of_sound=1540.0)
+    m = fs.MediumParams(attenuation=0.3)
+    pf = fs.pfield(g, d, p, m).cpu().numpy()
+    assert pf.shape == g.shape[:-1]
+    pl = O.pfield_plan(g[..., 0].ravel(), g[..., 1].ravel(), d, p, m)
+    want = O.pfield_oracle(g[..., 0].ravel(), g[..., 1].ravel(), d, p, m, pl).reshape(g.shape[:-1])
+    assert rel_l2(pf, want) < FP32_TOL
+
+
+def test_pfield_convex_out_of_field(cuda):
+    p = fs.C5_2v()
+    g = f32(np.array([[0.0, 0.03], [0.0, 0.001], [0.01, 0.04], [0.0, -0.01]]))
+    pf = fs.pfield(g, np.zeros(128), p).cpu().numpy()
+    pl = O.pfield_plan(g[:, 0], g[:, 1], np.zeros(128), p, fs.MediumParams())
+    want = O.pfield_oracle(g[:, 0], g[:, 1], np.zeros(128), p, fs.MediumParams(), pl)
+    assert pf[3] == 0.0 and pf[1] == 0.0  # behind the array / inside the arc
+    assert rel_l2(pf, want) < FP32_TOL
