@@ -132,6 +132,74 @@ __device__ __forceinline__ float dir_at(const Problem& p, const Geo& g, double f
   return sinc_rad(p.kappa_per_hz * (float)f * g.sint);
 }
 
+// ---------------------------------------------------------------- fast variants
+// Same quantities as geometry() / green_at(), evaluated with the gfx950
+// hardware transcendentals: v_sin_f32 / v_cos_f32 take revolutions (max abs
+// error 1.3e-7 measured over [0, 1)), v_exp_f32 (rel 8e-8), v_rsq_f64 refined
+// by one Newton step (rel ~1e-15). The phase is still reduced in FP64.
+constexpr float kLog2e = 1.4426950408889634f;
+
+__device__ __forceinline__ float sinc_fast(float a) {
+  // sin(a)/a: Taylor to a^8 below |a| = 1 (error < 3e-8), hardware sine above.
+  const float a2 = a * a;
+  if (fabsf(a) < 1.0f) {
+    return fmaf(a2, fmaf(a2, fmaf(a2, fmaf(a2, 2.7557319e-6f, -1.9841270e-4f), 8.3333333e-3f), -0.16666667f), 1.0f);
+  }
+  return __builtin_amdgcn_sinf(a * (0.5f / kPi)) / a;
+}
+
+struct GeoF {
+  double t;    // r_c / c
+  float amp;   // obliquity * directivity / sqrt(r_c) / n_sub  (directivity at fc unless fulldir)
+  float att;   // Np / Hz, pre-multiplied by log2(e)
+  float sint;  // sin(theta)
+};
+
+__device__ __forceinline__ GeoF geometry_fast(const Problem& p, double xs, double zs, int e, int j) {
+  const int idx = e * p.NS + j;
+  const double dx = xs - p.sub_x[idx];
+  const double dz = zs - p.sub_z[idx];
+  const double r2 = fma(dx, dx, dz * dz);
+  const bool zero = !(r2 > 0.0);
+  double y = __builtin_amdgcn_rsq(zero ? 1.0 : r2);
+  y = y * fma(-0.5 * r2, y * y, 1.5);  // Newton step on 1/sqrt
+  const double r = zero ? 0.0 : r2 * y;
+  const double rcl = fmax(r, p.min_dist);
+  const float rinv = (float)y;
+  const float sa = zero ? 1.0f : (float)dx * rinv;
+  const float ca = zero ? 0.0f : fabsf((float)dz) * rinv;
+  const float ce = (float)p.ecos[e], se = (float)p.esin[e];
+  const float st = sa * ce - ca * se;
+  const float ct = ca * ce + sa * se;
+  float obl = (p.baffle == FS_BAFFLE_SOFT) ? ct : (p.baffle == FS_BAFFLE_RIGID) ? 1.0f : ct / (ct + p.baffle_b);
+  if (!(ct > 0.0f)) obl = 1e-16f;
+  const float rcf = (float)rcl;
+  float amp = obl * __builtin_amdgcn_rsqf(rcf) * p.inv_nsub;
+  if (!p.fulldir) amp *= sinc_fast(p.kappa * st);
+  GeoF g;
+  g.t = rcl * p.inv_c;
+  g.amp = amp;
+  g.att = p.att_coef * rcf * kLog2e;
+  g.sint = st;
+  return g;
+}
+
+// amp exp(-att f) exp(i 2 pi frac(f t)) with hardware sin/cos/exp2
+__device__ __forceinline__ cf green_fast(const GeoF& g, double f, float amp) {
+  const double nu = f * g.t;
+  const float ph = (float)(nu - floor(nu));
+  const float m = amp * __builtin_amdgcn_exp2f(-g.att * (float)f);
+  return {m * __builtin_amdgcn_cosf(ph), m * __builtin_amdgcn_sinf(ph)};
+}
+
+// per-step multiplier exp((-att + i 2 pi t) df) with hardware sin/cos/exp2
+__device__ __forceinline__ cf green_step_fast(const GeoF& g, double df) {
+  const double nu = df * g.t;
+  const float ph = (float)(nu - floor(nu));
+  const float m = __builtin_amdgcn_exp2f(-g.att * (float)df);
+  return {m * __builtin_amdgcn_cosf(ph), m * __builtin_amdgcn_sinf(ph)};
+}
+
 int timing_start(hipStream_t st);
 void timing_stop(int slot, hipStream_t st);
 

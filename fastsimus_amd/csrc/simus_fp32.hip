@@ -85,12 +85,12 @@ __global__ __launch_bounds__(256) void k_tx(Problem p, int tx, int nfp, int fb0,
   const cf* At = A + (size_t)tx * p.E * nfp + k0;
   for (int e = 0; e < p.E; ++e) {
     cf g[NSUB], q[NSUB];
-    Geo geo[NSUB];
+    GeoF geo[NSUB];
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
-      geo[j] = geometry(p, xs, zs, e, j);
-      g[j] = green_at(geo[j], fstart);
-      q[j] = green_step(geo[j], p.df);
+      geo[j] = geometry_fast(p, xs, zs, e, j);
+      g[j] = green_fast(geo[j], fstart, geo[j].amp);
+      q[j] = green_step_fast(geo[j], p.df);
     }
     const cf* Ae = At + (size_t)e * nfp;
 #pragma unroll
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void k_tx(Problem p, int tx, int nfp, int fb0,
 #pragma unroll
       for (int j = 0; j < NSUB; ++j) {
         if constexpr (FULLDIR) {
-          const float d = dir_at(p, geo[j], fstart + (double)k * p.df);
+          const float d = sinc_fast(p.kappa_per_hz * (float)(fstart + (double)k * p.df) * geo[j].sint);
           G.x = fmaf(g[j].x, d, G.x);
           G.y = fmaf(g[j].y, d, G.y);
         } else {
@@ -166,12 +166,12 @@ __global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_
     const float xs = p.x[s], zs = p.z[s];
     if (out_of_field(p, xs, zs)) continue;
     cf g[NSUB], q[NSUB];
-    Geo geo[NSUB];
+    GeoF geo[NSUB];
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
-      geo[j] = geometry(p, xs, zs, ee, j);
-      g[j] = green_at(geo[j], fstart);
-      q[j] = green_step(geo[j], p.df);
+      geo[j] = geometry_fast(p, xs, zs, ee, j);
+      g[j] = green_fast(geo[j], fstart, geo[j].amp);
+      q[j] = green_step_fast(geo[j], p.df);
     }
 #pragma unroll
     for (int k = 0; k < FB; ++k) {
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_
 #pragma unroll
       for (int j = 0; j < NSUB; ++j) {
         if constexpr (FULLDIR) {
-          const float d = dir_at(p, geo[j], fstart + (double)k * p.df);
+          const float d = sinc_fast(p.kappa_per_hz * (float)(fstart + (double)k * p.df) * geo[j].sint);
           G.x = fmaf(g[j].x, d, G.x);
           G.y = fmaf(g[j].y, d, G.y);
         } else {

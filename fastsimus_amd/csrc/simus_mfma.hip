@@ -70,46 +70,70 @@ __device__ __forceinline__ floatx4 mma(u32x4 a, u32x4 b, floatx4 c) {
 }
 
 // ---------------------------------------------------------------- tile jobs
-// Upper-triangular tile pairs (a <= b) of an ET x ET grid, split into four
-// groups: L x L, U x U, L0 x U, L1 x U with L = [0, ET/2), U = [ET/2, ET),
-// L0/L1 the two halves of L.
-template <int ET>
-struct Jobs {
-  static constexpr int H = ET / 2;
-  static constexpr int H0 = H / 2;
-  __host__ __device__ static constexpr int count(int w) {
-    return w == 0 ? H * (H + 1) / 2 : w == 1 ? (ET - H) * (ET - H + 1) / 2 : w == 2 ? H0 * (ET - H) : (H - H0) * (ET - H);
-  }
-  __host__ __device__ static constexpr void pair(int w, int j, int& a, int& b) {
-    if (w == 0 || w == 1) {
-      const int base = (w == 0) ? 0 : H;
-      const int n = (w == 0) ? H : ET - H;
-      int r = 0, jj = j;
-      while (jj >= n - r) {
-        jj -= n - r;
-        ++r;
-      }
-      a = base + r;
-      b = base + r + jj;
-    } else {
-      const int r0 = (w == 2) ? 0 : H0;
-      const int nb = ET - H;
-      a = r0 + j / nb;
-      b = H + j % nb;
-    }
-  }
-  __host__ __device__ static constexpr int row_lo(int w) { return w == 0 ? 0 : w == 1 ? H : w == 2 ? 0 : H0; }
-  __host__ __device__ static constexpr int row_hi(int w) { return w == 0 ? H : w == 1 ? ET : w == 2 ? H0 : H; }
-  __host__ __device__ static constexpr int col_lo(int w) { return w == 0 ? 0 : H; }
-  __host__ __device__ static constexpr int col_hi(int w) { return w == 0 ? H : ET; }
-  static constexpr int max_count() {
-    int m = 0;
-    for (int w = 0; w < 4; ++w) m = count(w) > m ? count(w) : m;
-    return m;
-  }
+// Upper-triangular 16x16 tile pairs (a <= b) of the E x E matrix, dealt to the
+// four waves of a workgroup: 9 pairs each for 8 tiles (E <= 128), 3/3/2/2 for
+// 4 tiles (E <= 64), grouped so that each wave touches few distinct rows/cols.
+template <int ET, int W>
+struct JobList;
+template <> struct JobList<8, 0> {
+  static constexpr int n = 9;
+  static constexpr int a[9] = {0, 0, 0, 0, 1, 1, 1, 2, 2};
+  static constexpr int b[9] = {0, 1, 2, 3, 1, 2, 3, 2, 3};
+};
+template <> struct JobList<8, 1> {
+  static constexpr int n = 9;
+  static constexpr int a[9] = {4, 4, 4, 5, 5, 5, 6, 6, 7};
+  static constexpr int b[9] = {5, 6, 7, 5, 6, 7, 6, 7, 7};
+};
+template <> struct JobList<8, 2> {
+  static constexpr int n = 9;
+  static constexpr int a[9] = {0, 0, 0, 0, 1, 1, 1, 1, 4};
+  static constexpr int b[9] = {4, 5, 6, 7, 4, 5, 6, 7, 4};
+};
+template <> struct JobList<8, 3> {
+  static constexpr int n = 9;
+  static constexpr int a[9] = {2, 2, 2, 2, 3, 3, 3, 3, 3};
+  static constexpr int b[9] = {4, 5, 6, 7, 3, 4, 5, 6, 7};
+};
+template <> struct JobList<4, 0> {
+  static constexpr int n = 3;
+  static constexpr int a[3] = {0, 0, 1};
+  static constexpr int b[3] = {0, 1, 1};
+};
+template <> struct JobList<4, 1> {
+  static constexpr int n = 3;
+  static constexpr int a[3] = {2, 2, 3};
+  static constexpr int b[3] = {2, 3, 3};
+};
+template <> struct JobList<4, 2> {
+  static constexpr int n = 2;
+  static constexpr int a[2] = {0, 0};
+  static constexpr int b[2] = {2, 3};
+};
+template <> struct JobList<4, 3> {
+  static constexpr int n = 2;
+  static constexpr int a[2] = {1, 1};
+  static constexpr int b[2] = {2, 3};
 };
 
+template <int ET, int W>
+__host__ __device__ constexpr bool uses_row(int r) {
+  for (int j = 0; j < JobList<ET, W>::n; ++j)
+    if (JobList<ET, W>::a[j] == r) return true;
+  return false;
+}
+template <int ET, int W>
+__host__ __device__ constexpr bool uses_col(int c) {
+  for (int j = 0; j < JobList<ET, W>::n; ++j)
+    if (JobList<ET, W>::b[j] == c) return true;
+  return false;
+}
+
 // ---------------------------------------------------------------- G fragments
+// Thread (element group el, scatterer quad g): EPT elements x 4 scatterers,
+// every (s, e, j) term evaluated directly at the FBK frequencies of the block
+// (FP64 phase reduction, hardware sin/cos/exp2), summed over sub-elements,
+// scaled by sqrt|rc| and written as split-bf16 fragments.
 template <int NSUB, int EPT, int FBK, bool FULLDIR, int ET>
 __device__ __forceinline__ void gen_fragments(const Problem& p, int s0, int s_end, int fb, u32x4* lds) {
   using L = FragLds<ET, FBK>;
@@ -117,7 +141,7 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, int s0, int s_en
   const int el = threadIdx.x & 63;  // element group
   float w[4];
   unsigned m01 = 0u, m23 = 0u;
-  double xs[4], zs[4];
+  float xs[4], zs[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int s = s0 + 4 * g + i;
@@ -141,55 +165,42 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, int s0, int s_en
     const int e = el * EPT + u;
     const bool ev = e < p.E;
     const int ee = ev ? e : 0;
-    cf gs[4][NSUB], qs[4][NSUB];
-    float st[4][NSUB];
+    float re[FBK][4], im[FBK][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
+      for (int k = 0; k < FBK; ++k) re[k][i] = im[k][i] = 0.f;
+      const float wi = ev ? w[i] : 0.f;
+#pragma unroll
       for (int j = 0; j < NSUB; ++j) {
-        const Geo geo = geometry(p, xs[i], zs[i], ee, j);
-        gs[i][j] = green_at(geo, fstart);
-        qs[i][j] = green_step(geo, p.df);
-        st[i][j] = geo.sint;
+        const GeoF geo = geometry_fast(p, xs[i], zs[i], ee, j);
+        const float a = geo.amp * wi;
+#pragma unroll
+        for (int k = 0; k < FBK; ++k) {
+          const double f = fstart + (double)k * p.df;
+          float ak = a;
+          if constexpr (FULLDIR) ak *= sinc_fast(p.kappa_per_hz * (float)f * geo.sint);
+          const cf v = green_fast(geo, f, ak);
+          re[k][i] += v.x;
+          im[k][i] += v.y;
+        }
       }
     }
     const int tile = e >> 4;
     const int lane = (e & 15) + 16 * g;
+    const u32x4 msk = {m01, m23, m01, m23};
 #pragma unroll
     for (int k = 0; k < FBK; ++k) {
-      float re[4], im[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        cf G = {0.f, 0.f};
-#pragma unroll
-        for (int j = 0; j < NSUB; ++j) {
-          if constexpr (FULLDIR) {
-            const float d = sinc_rad(p.kappa_per_hz * (float)(fstart + (double)k * p.df) * st[i][j]);
-            G.x = fmaf(gs[i][j].x, d, G.x);
-            G.y = fmaf(gs[i][j].y, d, G.y);
-          } else {
-            G.x += gs[i][j].x;
-            G.y += gs[i][j].y;
-          }
-          if (k + 1 < FBK) gs[i][j] = cmul(gs[i][j], qs[i][j]);
-        }
-        const float wi = ev ? w[i] : 0.f;
-        re[i] = G.x * wi;
-        im[i] = G.y * wi;
-      }
       unsigned h0, h1, h2, h3, l0, l1, l2, l3;
-      split_pair(re[0], re[1], h0, l0);
-      split_pair(re[2], re[3], h1, l1);
-      split_pair(im[0], im[1], h2, l2);
-      split_pair(im[2], im[3], h3, l3);
+      split_pair(re[k][0], re[k][1], h0, l0);
+      split_pair(re[k][2], re[k][3], h1, l1);
+      split_pair(im[k][0], im[k][1], h2, l2);
+      split_pair(im[k][2], im[k][3], h3, l3);
       const u32x4 hi = {h0, h1, h2, h3}, lo = {l0, l1, l2, l3};
-      if (tile < ET) {
-        lds[L::idx(k, 0, 0, tile, lane)] = hi;
-        lds[L::idx(k, 0, 1, tile, lane)] = lo;
-        const u32x4 msk = {m01, m23, m01, m23};
-        lds[L::idx(k, 1, 0, tile, lane)] = hi ^ msk;
-        lds[L::idx(k, 1, 1, tile, lane)] = lo ^ msk;
-      }
+      lds[L::idx(k, 0, 0, tile, lane)] = hi;
+      lds[L::idx(k, 0, 1, tile, lane)] = lo;
+      lds[L::idx(k, 1, 0, tile, lane)] = hi ^ msk;
+      lds[L::idx(k, 1, 1, tile, lane)] = lo ^ msk;
     }
   }
 }
@@ -197,55 +208,56 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, int s0, int s_en
 // ---------------------------------------------------------------- MFMA tiles of one wave
 template <int ET, int FBK, int W>
 struct WaveTiles {
-  using J = Jobs<ET>;
-  static constexpr int NJ = J::count(W);
-  static constexpr int R0 = J::row_lo(W), R1 = J::row_hi(W), C0 = J::col_lo(W), C1 = J::col_hi(W);
-  static constexpr int NR = R1 - R0, NC = C1 - C0;
-  floatx4 acc[FBK][NJ > 0 ? NJ : 1][2];
+  using J = JobList<ET, W>;
+  static constexpr int NJ = J::n;
+  floatx4 acc[FBK][NJ][2];
 
   __device__ __forceinline__ void zero() {
 #pragma unroll
     for (int k = 0; k < FBK; ++k)
 #pragma unroll
-      for (int j = 0; j < (NJ > 0 ? NJ : 1); ++j) acc[k][j][0] = acc[k][j][1] = (floatx4){0.f, 0.f, 0.f, 0.f};
+      for (int j = 0; j < NJ; ++j) acc[k][j][0] = acc[k][j][1] = (floatx4){0.f, 0.f, 0.f, 0.f};
   }
 
+  // One 16-scatterer tile: for every frequency of the block, load the sign-applied
+  // column fragments this wave needs, then stream its rows.
   __device__ __forceinline__ void step(const u32x4* lds) {
     using L = FragLds<ET, FBK>;
     const int lane = threadIdx.x & 63;
     const u32x4 nim = {0u, 0u, 0x80008000u, 0x80008000u};
 #pragma unroll
     for (int k = 0; k < FBK; ++k) {
-      u32x4 arh[NR > 0 ? NR : 1], arl[NR > 0 ? NR : 1], aih[NR > 0 ? NR : 1], ail[NR > 0 ? NR : 1];
-      u32x4 bh[NC > 0 ? NC : 1], bl[NC > 0 ? NC : 1];
+      u32x4 bh[ET], bl[ET];
 #pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        const u32x4 h = lds[L::idx(k, 0, 0, R0 + r, lane)];
-        const u32x4 l = lds[L::idx(k, 0, 1, R0 + r, lane)];
-        arh[r] = h ^ nim;
-        arl[r] = l ^ nim;
-        aih[r] = (u32x4){h.z, h.w, h.x, h.y};
-        ail[r] = (u32x4){l.z, l.w, l.x, l.y};
+      for (int c = 0; c < ET; ++c) {
+        if constexpr (true) {
+          if (uses_col<ET, W>(c)) {
+            bh[c] = lds[L::idx(k, 1, 0, c, lane)];
+            bl[c] = lds[L::idx(k, 1, 1, c, lane)];
+          }
+        }
       }
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        bh[c] = lds[L::idx(k, 1, 0, C0 + c, lane)];
-        bl[c] = lds[L::idx(k, 1, 1, C0 + c, lane)];
-      }
+      for (int r = 0; r < ET; ++r) {
+        if (!uses_row<ET, W>(r)) continue;
+        const u32x4 h = lds[L::idx(k, 0, 0, r, lane)];
+        const u32x4 l = lds[L::idx(k, 0, 1, r, lane)];
+        const u32x4 arh = h ^ nim, arl = l ^ nim;
+        const u32x4 aih = {h.z, h.w, h.x, h.y}, ail = {l.z, l.w, l.x, l.y};
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) {
-        int a = 0, b = 0;
-        J::pair(W, j, a, b);
-        const int r = a - R0, c = b - C0;
-        floatx4 kr = acc[k][j][0], ki = acc[k][j][1];
-        kr = mma(arh[r], bh[c], kr);
-        ki = mma(aih[r], bh[c], ki);
-        kr = mma(arh[r], bl[c], kr);
-        ki = mma(aih[r], bl[c], ki);
-        kr = mma(arl[r], bh[c], kr);
-        ki = mma(ail[r], bh[c], ki);
-        acc[k][j][0] = kr;
-        acc[k][j][1] = ki;
+        for (int j = 0; j < NJ; ++j) {
+          if (J::a[j] != r) continue;
+          const int c = J::b[j];
+          floatx4 kr = acc[k][j][0], ki = acc[k][j][1];
+          kr = mma(arh, bh[c], kr);
+          ki = mma(aih, bh[c], ki);
+          kr = mma(arh, bl[c], kr);
+          ki = mma(aih, bl[c], ki);
+          kr = mma(arl, bh[c], kr);
+          ki = mma(ail, bh[c], ki);
+          acc[k][j][0] = kr;
+          acc[k][j][1] = ki;
+        }
       }
     }
   }
@@ -258,12 +270,10 @@ struct WaveTiles {
       cf* Kk = kpart + (size_t)(kbase + k) * E_pad * E_pad;
 #pragma unroll
       for (int j = 0; j < NJ; ++j) {
-        int a = 0, b = 0;
-        J::pair(W, j, a, b);
-        const int col = 16 * b + (lane & 15);
+        const int col = 16 * J::b[j] + (lane & 15);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int row = 16 * a + 4 * (lane >> 4) + i;
+          const int row = 16 * J::a[j] + 4 * (lane >> 4) + i;
           Kk[(size_t)row * E_pad + col] = {acc[k][j][0][i], acc[k][j][1][i]};
         }
       }
@@ -286,7 +296,7 @@ __device__ __forceinline__ void msm_wave(const Problem& p, int sb, int se, int f
 }
 
 template <int NSUB, int ET, int FBK, bool FULLDIR>
-__global__ __launch_bounds__(256) void k_msm(Problem p, int s_per_chunk, int nfpad, cf* __restrict__ kpart) {
+__global__ __launch_bounds__(256, (NSUB <= 2 && !FULLDIR) ? 2 : 1) void k_msm(Problem p, int s_per_chunk, int nfpad, cf* __restrict__ kpart) {
   extern __shared__ u32x4 lds_frag[];
   const int fb = blockIdx.x;
   const int chunk = blockIdx.y;
