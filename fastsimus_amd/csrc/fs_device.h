@@ -145,7 +145,7 @@ __device__ __forceinline__ float sinc_fast(float a) {
   if (fabsf(a) < 1.0f) {
     return fmaf(a2, fmaf(a2, fmaf(a2, fmaf(a2, 2.7557319e-6f, -1.9841270e-4f), 8.3333333e-3f), -0.16666667f), 1.0f);
   }
-  return __builtin_amdgcn_sinf(a * (0.5f / kPi)) / a;
+  return __builtin_amdgcn_sinf(a * (0.5f / kPi)) * __builtin_amdgcn_rcpf(a);
 }
 
 struct GeoF {

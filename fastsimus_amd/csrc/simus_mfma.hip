@@ -70,9 +70,10 @@ __device__ __forceinline__ floatx4 mma(u32x4 a, u32x4 b, floatx4 c) {
 }
 
 // ---------------------------------------------------------------- tile jobs
-// Upper-triangular 16x16 tile pairs (a <= b) of the E x E matrix, dealt to the
-// four waves of a workgroup: 9 pairs each for 8 tiles (E <= 128), 3/3/2/2 for
-// 4 tiles (E <= 64), grouped so that each wave touches few distinct rows/cols.
+// Upper-triangular 16x16 tile pairs (a <= b) of the E x E matrix. A frequency
+// pair of the block is shared by WPP waves, each owning one list: 9 pairs
+// each for 8 tiles (E <= 128, WPP = 4), 5 each for 4 tiles (E <= 64,
+// WPP = 2), grouped so that each wave touches few distinct rows/cols.
 template <int ET, int W>
 struct JobList;
 template <> struct JobList<8, 0> {
@@ -96,24 +97,14 @@ template <> struct JobList<8, 3> {
   static constexpr int b[9] = {4, 5, 6, 7, 3, 4, 5, 6, 7};
 };
 template <> struct JobList<4, 0> {
-  static constexpr int n = 3;
-  static constexpr int a[3] = {0, 0, 1};
-  static constexpr int b[3] = {0, 1, 1};
+  static constexpr int n = 5;
+  static constexpr int a[5] = {0, 0, 0, 0, 1};
+  static constexpr int b[5] = {0, 1, 2, 3, 1};
 };
 template <> struct JobList<4, 1> {
-  static constexpr int n = 3;
-  static constexpr int a[3] = {2, 2, 3};
-  static constexpr int b[3] = {2, 3, 3};
-};
-template <> struct JobList<4, 2> {
-  static constexpr int n = 2;
-  static constexpr int a[2] = {0, 0};
-  static constexpr int b[2] = {2, 3};
-};
-template <> struct JobList<4, 3> {
-  static constexpr int n = 2;
-  static constexpr int a[2] = {1, 1};
-  static constexpr int b[2] = {2, 3};
+  static constexpr int n = 5;
+  static constexpr int a[5] = {1, 1, 2, 2, 3};
+  static constexpr int b[5] = {2, 3, 2, 3, 3};
 };
 
 template <int ET, int W>
@@ -129,112 +120,183 @@ __host__ __device__ constexpr bool uses_col(int c) {
   return false;
 }
 
+// Workgroup shape: one thread per (scatterer quad, element) of a tile, i.e.
+// ET waves; FBK = 4 frequencies per block, two per wave, WPP waves per pair.
+template <int ET>
+struct MsmShape {
+  static constexpr int FBK = 4;
+  static constexpr int NW = ET;                 // waves per workgroup
+  static constexpr int THREADS = 64 * NW;
+  static constexpr int WPP = NW * 2 / FBK;      // waves sharing one frequency pair
+  static constexpr int WG_PER_CU = ET == 8 ? 1 : 2;
+};
+
+// sin(a)/a for |a| <= pi: Taylor series to a^16 (truncation < 1e-8 on that range).
+__device__ __forceinline__ float sinc_poly_pi(float a) {
+  const float x = a * a;
+  float r = fmaf(x, -2.8114572e-15f, 7.6471637e-13f);
+  r = fmaf(x, r, -1.6059044e-10f);
+  r = fmaf(x, r, 2.5052108e-8f);
+  r = fmaf(x, r, -2.7557319e-6f);
+  r = fmaf(x, r, 1.9841270e-4f);
+  r = fmaf(x, r, -8.3333333e-3f);
+  r = fmaf(x, r, 0.16666667f);
+  return fmaf(-x, r, 1.0f);
+}
+
+// Per-thread element data, constant over the whole launch.
+template <int NSUB>
+struct ElemData {
+  double sx[NSUB], sz[NSUB];
+  float ce, se;
+  bool valid;
+};
+
+template <int NSUB, int ET>
+__device__ __forceinline__ ElemData<NSUB> load_element(const Problem& p) {
+  ElemData<NSUB> d;
+  const int e = threadIdx.x % (ET * 16);
+  d.valid = e < p.E;
+  const int ee = d.valid ? e : 0;
+#pragma unroll
+  for (int j = 0; j < NSUB; ++j) {
+    d.sx[j] = p.sub_x[ee * NSUB + j];
+    d.sz[j] = p.sub_z[ee * NSUB + j];
+  }
+  d.ce = (float)p.ecos[ee];
+  d.se = (float)p.esin[ee];
+  return d;
+}
+
 // ---------------------------------------------------------------- G fragments
-// Thread (element group el, scatterer quad g): EPT elements x 4 scatterers,
-// every (s, e, j) term evaluated directly at the FBK frequencies of the block
-// (FP64 phase reduction, hardware sin/cos/exp2), summed over sub-elements,
-// scaled by sqrt|rc| and written as split-bf16 fragments.
-template <int NSUB, int EPT, int FBK, bool FULLDIR, int ET>
-__device__ __forceinline__ void gen_fragments(const Problem& p, int s0, int s_end, int fb, u32x4* lds) {
+// Thread (scatterer quad g, element e): for each of the quad's 4 scatterers
+// and each sub-element, the frequency-independent geometry once per block
+// (FP64 distance and phase reduction), the seed g(f_b) and the step
+// q = exp((-att + i 2 pi t) df), then the block's FBK frequencies by the FP32
+// recurrence g <- g q. Sub-elements are summed, sqrt|rc| is folded into the
+// amplitude, and every frequency is written as split-bf16 fragments
+// (T1 plain, T2 with the sign of rc applied).
+template <int NSUB, int FBK, bool FULLDIR, int ET>
+__device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<NSUB>& ed, bool poly_ok, int s0,
+                                              int s_end, double fb_hz, u32x4* lds) {
   using L = FragLds<ET, FBK>;
-  const int g = threadIdx.x >> 6;   // scatterer quad of the tile
-  const int el = threadIdx.x & 63;  // element group
-  float w[4];
+  const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / (ET * 16)));  // wave-uniform
+  const int e = threadIdx.x % (ET * 16);
+  cf gk[4][NSUB], q[4][NSUB];
+  float sth[4][NSUB];
   unsigned m01 = 0u, m23 = 0u;
-  float xs[4], zs[4];
+  const float fbf = (float)fb_hz, dff = (float)p.df;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int s = s0 + 4 * g + i;
     const bool valid = s < s_end;
-    const float x = valid ? p.x[s] : 0.f;
-    const float z = valid ? p.z[s] : 1e-2f;
-    const float r = valid ? p.rc[s] : 0.f;
-    const bool live = valid && r != 0.f && !out_of_field(p, x, z);
-    w[i] = live ? sqrtf(fabsf(r)) : 0.f;
-    const unsigned sb = (r < 0.f) ? 1u : 0u;
+    const int sc = valid ? s : s_end - 1;  // in-bounds load, masked below
+    const float x = p.x[sc], z = p.z[sc], r = p.rc[sc];
+    const bool live = valid && ed.valid && r != 0.f && !out_of_field(p, x, z);
+    const float w = live ? __builtin_amdgcn_sqrtf(fabsf(r)) * p.inv_nsub : 0.f;
+    const unsigned sb = (live && r < 0.f) ? 1u : 0u;
     if (i < 2)
       m01 |= sb << (15 + 16 * i);
     else
       m23 |= sb << (15 + 16 * (i - 2));
-    xs[i] = x;
-    zs[i] = z;
-  }
-  const double fstart = p.f0 + (double)(fb * FBK) * p.df;
+    const double xd = (double)x, zd = (double)z;
 #pragma unroll
-  for (int u = 0; u < EPT; ++u) {
-    const int e = el * EPT + u;
-    const bool ev = e < p.E;
-    const int ee = ev ? e : 0;
-    float re[FBK][4], im[FBK][4];
+    for (int j = 0; j < NSUB; ++j) {
+      const double dx = xd - ed.sx[j];
+      const double dz = zd - ed.sz[j];
+      const double r2 = fma(dx, dx, dz * dz);
+      const bool zero = !(r2 > 0.0);
+      double y = __builtin_amdgcn_rsq(zero ? 1.0 : r2);
+      y = y * fma(-0.5 * r2, y * y, 1.5);  // Newton step on 1/sqrt
+      const double rcl = fmax(zero ? 0.0 : r2 * y, p.min_dist);
+      const float rinv = (float)y;
+      const float sa = zero ? 1.0f : (float)dx * rinv;
+      const float ca = zero ? 0.0f : fabsf((float)dz) * rinv;
+      const float st = sa * ed.ce - ca * ed.se;
+      const float ct = ca * ed.ce + sa * ed.se;
+      float obl = (p.baffle == FS_BAFFLE_SOFT)    ? ct
+                  : (p.baffle == FS_BAFFLE_RIGID) ? 1.0f
+                                                  : ct * __builtin_amdgcn_rcpf(ct + p.baffle_b);
+      if (!(ct > 0.0f)) obl = 1e-16f;
+      const float rcf = (float)rcl;
+      float amp = obl * __builtin_amdgcn_rsqf(rcf) * w;
+      if constexpr (!FULLDIR) amp *= poly_ok ? sinc_poly_pi(p.kappa * st) : sinc_fast(p.kappa * st);
+      sth[i][j] = st;
+      const double t = rcl * p.inv_c;
+      const float att = p.att_coef * rcf * kLog2e;
+      const double nu0 = fb_hz * t, nud = p.df * t;
+      const float ph0 = (float)(nu0 - floor(nu0)), phd = (float)(nud - floor(nud));
+      const float m0 = amp * __builtin_amdgcn_exp2f(-att * fbf);
+      const float md = __builtin_amdgcn_exp2f(-att * dff);
+      gk[i][j] = {m0 * __builtin_amdgcn_cosf(ph0), m0 * __builtin_amdgcn_sinf(ph0)};
+      q[i][j] = {md * __builtin_amdgcn_cosf(phd), md * __builtin_amdgcn_sinf(phd)};
+    }
+  }
+  const int tile = e >> 4;
+  const int lane = (e & 15) + 16 * g;
+  const u32x4 msk = {m01, m23, m01, m23};
+#pragma unroll
+  for (int k = 0; k < FBK; ++k) {
+    float re[4], im[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-#pragma unroll
-      for (int k = 0; k < FBK; ++k) re[k][i] = im[k][i] = 0.f;
-      const float wi = ev ? w[i] : 0.f;
+      re[i] = im[i] = 0.f;
 #pragma unroll
       for (int j = 0; j < NSUB; ++j) {
-        const GeoF geo = geometry_fast(p, xs[i], zs[i], ee, j);
-        const float a = geo.amp * wi;
-#pragma unroll
-        for (int k = 0; k < FBK; ++k) {
-          const double f = fstart + (double)k * p.df;
-          float ak = a;
-          if constexpr (FULLDIR) ak *= sinc_fast(p.kappa_per_hz * (float)f * geo.sint);
-          const cf v = green_fast(geo, f, ak);
-          re[k][i] += v.x;
-          im[k][i] += v.y;
+        if constexpr (FULLDIR) {
+          const float d = sinc_fast(p.kappa_per_hz * (fbf + (float)k * dff) * sth[i][j]);
+          re[i] = fmaf(d, gk[i][j].x, re[i]);
+          im[i] = fmaf(d, gk[i][j].y, im[i]);
+        } else {
+          re[i] += gk[i][j].x;
+          im[i] += gk[i][j].y;
         }
+        if (k + 1 < FBK) gk[i][j] = cmul(gk[i][j], q[i][j]);
       }
     }
-    const int tile = e >> 4;
-    const int lane = (e & 15) + 16 * g;
-    const u32x4 msk = {m01, m23, m01, m23};
-#pragma unroll
-    for (int k = 0; k < FBK; ++k) {
-      unsigned h0, h1, h2, h3, l0, l1, l2, l3;
-      split_pair(re[k][0], re[k][1], h0, l0);
-      split_pair(re[k][2], re[k][3], h1, l1);
-      split_pair(im[k][0], im[k][1], h2, l2);
-      split_pair(im[k][2], im[k][3], h3, l3);
-      const u32x4 hi = {h0, h1, h2, h3}, lo = {l0, l1, l2, l3};
-      lds[L::idx(k, 0, 0, tile, lane)] = hi;
-      lds[L::idx(k, 0, 1, tile, lane)] = lo;
-      lds[L::idx(k, 1, 0, tile, lane)] = hi ^ msk;
-      lds[L::idx(k, 1, 1, tile, lane)] = lo ^ msk;
-    }
+    unsigned h0, h1, h2, h3, l0, l1, l2, l3;
+    split_pair(re[0], re[1], h0, l0);
+    split_pair(re[2], re[3], h1, l1);
+    split_pair(im[0], im[1], h2, l2);
+    split_pair(im[2], im[3], h3, l3);
+    const u32x4 hi = {h0, h1, h2, h3}, lo = {l0, l1, l2, l3};
+    lds[L::idx(k, 0, 0, tile, lane)] = hi;
+    lds[L::idx(k, 0, 1, tile, lane)] = lo;
+    lds[L::idx(k, 1, 0, tile, lane)] = hi ^ msk;
+    lds[L::idx(k, 1, 1, tile, lane)] = lo ^ msk;
   }
 }
 
 // ---------------------------------------------------------------- MFMA tiles of one wave
+// Accumulators of one wave: its job list at the two frequencies k0, k0 + 1 of the block.
 template <int ET, int FBK, int W>
 struct WaveTiles {
   using J = JobList<ET, W>;
   static constexpr int NJ = J::n;
-  floatx4 acc[FBK][NJ][2];
+  floatx4 acc[2][NJ][2];
 
   __device__ __forceinline__ void zero() {
 #pragma unroll
-    for (int k = 0; k < FBK; ++k)
+    for (int k = 0; k < 2; ++k)
 #pragma unroll
       for (int j = 0; j < NJ; ++j) acc[k][j][0] = acc[k][j][1] = (floatx4){0.f, 0.f, 0.f, 0.f};
   }
 
-  // One 16-scatterer tile: for every frequency of the block, load the sign-applied
+  // One 16-scatterer tile: for each of the wave's frequencies, load the sign-applied
   // column fragments this wave needs, then stream its rows.
-  __device__ __forceinline__ void step(const u32x4* lds) {
+  __device__ __forceinline__ void step(const u32x4* lds, int k0) {
     using L = FragLds<ET, FBK>;
     const int lane = threadIdx.x & 63;
     const u32x4 nim = {0u, 0u, 0x80008000u, 0x80008000u};
 #pragma unroll
-    for (int k = 0; k < FBK; ++k) {
+    for (int kk = 0; kk < 2; ++kk) {
+      const int k = k0 + kk;
       u32x4 bh[ET], bl[ET];
 #pragma unroll
       for (int c = 0; c < ET; ++c) {
-        if constexpr (true) {
-          if (uses_col<ET, W>(c)) {
-            bh[c] = lds[L::idx(k, 1, 0, c, lane)];
-            bl[c] = lds[L::idx(k, 1, 1, c, lane)];
-          }
+        if (uses_col<ET, W>(c)) {
+          bh[c] = lds[L::idx(k, 1, 0, c, lane)];
+          bl[c] = lds[L::idx(k, 1, 1, c, lane)];
         }
       }
 #pragma unroll
@@ -248,66 +310,82 @@ struct WaveTiles {
         for (int j = 0; j < NJ; ++j) {
           if (J::a[j] != r) continue;
           const int c = J::b[j];
-          floatx4 kr = acc[k][j][0], ki = acc[k][j][1];
+          floatx4 kr = acc[kk][j][0], ki = acc[kk][j][1];
           kr = mma(arh, bh[c], kr);
           ki = mma(aih, bh[c], ki);
           kr = mma(arh, bl[c], kr);
           ki = mma(aih, bl[c], ki);
           kr = mma(arl, bh[c], kr);
           ki = mma(ail, bh[c], ki);
-          acc[k][j][0] = kr;
-          acc[k][j][1] = ki;
+          acc[kk][j][0] = kr;
+          acc[kk][j][1] = ki;
         }
       }
     }
   }
 
   // Kpart[chunk][kglob][e][e'] (complex), D layout: lane -> (row 4*(lane/16)+i, col lane%16)
-  __device__ __forceinline__ void store(cf* kpart, int E_pad, int kbase) {
+  __device__ __forceinline__ void store(cf* kpart, int E_pad, int kglob0) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int k = 0; k < FBK; ++k) {
-      cf* Kk = kpart + (size_t)(kbase + k) * E_pad * E_pad;
+    for (int kk = 0; kk < 2; ++kk) {
+      cf* Kk = kpart + (size_t)(kglob0 + kk) * E_pad * E_pad;
 #pragma unroll
       for (int j = 0; j < NJ; ++j) {
         const int col = 16 * J::b[j] + (lane & 15);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int row = 16 * J::a[j] + 4 * (lane >> 4) + i;
-          Kk[(size_t)row * E_pad + col] = {acc[k][j][0][i], acc[k][j][1][i]};
+          Kk[(size_t)row * E_pad + col] = {acc[kk][j][0][i], acc[kk][j][1][i]};
         }
       }
     }
   }
 };
 
-template <int NSUB, int ET, int FBK, bool FULLDIR, int W>
-__device__ __forceinline__ void msm_wave(const Problem& p, int sb, int se, int fb, u32x4* lds, cf* kpart) {
-  constexpr int EPT = (ET * 16) / 64;
+template <int NSUB, int ET, bool FULLDIR, int W>
+__device__ __forceinline__ void msm_wave(const Problem& p, int sb, int se, int fb, int k0, u32x4* lds, cf* kpart) {
+  constexpr int FBK = MsmShape<ET>::FBK;
+  const ElemData<NSUB> ed = load_element<NSUB, ET>(p);
+  const bool poly_ok = !FULLDIR && p.kappa <= kPi;  // centre-frequency sinc argument |kappa sin| <= pi
+  const double fb_hz = p.f0 + (double)(fb * FBK) * p.df;
   WaveTiles<ET, FBK, W> t;
   t.zero();
   for (int s0 = sb; s0 < se; s0 += MST) {
-    gen_fragments<NSUB, EPT, FBK, FULLDIR, ET>(p, s0, se, fb, lds);
+    gen_fragments<NSUB, FBK, FULLDIR, ET>(p, ed, poly_ok, s0, se, fb_hz, lds);
     __syncthreads();
-    t.step(lds);
+    t.step(lds, k0);
     __syncthreads();
   }
-  t.store(kpart, ET * 16, fb * FBK);
+  t.store(kpart, ET * 16, fb * FBK + k0);
 }
 
-template <int NSUB, int ET, int FBK, bool FULLDIR>
-__global__ __launch_bounds__(256, (NSUB <= 2 && !FULLDIR) ? 2 : 1) void k_msm(Problem p, int s_per_chunk, int nfpad, cf* __restrict__ kpart) {
+// Grid (frequency block, scatterer chunk); ET waves per workgroup. Wave w owns
+// frequencies 2 (w / WPP) and 2 (w / WPP) + 1 of the block and job list w % WPP.
+template <int NSUB, int ET, bool FULLDIR>
+__global__ __launch_bounds__(MsmShape<ET>::THREADS, MsmShape<ET>::WG_PER_CU) void k_msm(Problem p, int s_per_chunk,
+                                                                                         int nfpad, cf* __restrict__ kpart) {
+  using SH = MsmShape<ET>;
   extern __shared__ u32x4 lds_frag[];
   const int fb = blockIdx.x;
   const int chunk = blockIdx.y;
   const int sb = chunk * s_per_chunk;
   const int se = min(sb + s_per_chunk, p.S);
   cf* kp = kpart + (size_t)chunk * nfpad * (ET * 16) * (ET * 16);
-  switch (threadIdx.x >> 6) {
-    case 0: msm_wave<NSUB, ET, FBK, FULLDIR, 0>(p, sb, se, fb, lds_frag, kp); break;
-    case 1: msm_wave<NSUB, ET, FBK, FULLDIR, 1>(p, sb, se, fb, lds_frag, kp); break;
-    case 2: msm_wave<NSUB, ET, FBK, FULLDIR, 2>(p, sb, se, fb, lds_frag, kp); break;
-    default: msm_wave<NSUB, ET, FBK, FULLDIR, 3>(p, sb, se, fb, lds_frag, kp); break;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int k0 = 2 * (w / SH::WPP);
+  if constexpr (SH::WPP == 4) {
+    switch (w % 4) {
+      case 0: msm_wave<NSUB, ET, FULLDIR, 0>(p, sb, se, fb, k0, lds_frag, kp); break;
+      case 1: msm_wave<NSUB, ET, FULLDIR, 1>(p, sb, se, fb, k0, lds_frag, kp); break;
+      case 2: msm_wave<NSUB, ET, FULLDIR, 2>(p, sb, se, fb, k0, lds_frag, kp); break;
+      default: msm_wave<NSUB, ET, FULLDIR, 3>(p, sb, se, fb, k0, lds_frag, kp); break;
+    }
+  } else {
+    if (w % 2 == 0)
+      msm_wave<NSUB, ET, FULLDIR, 0>(p, sb, se, fb, k0, lds_frag, kp);
+    else
+      msm_wave<NSUB, ET, FULLDIR, 1>(p, sb, se, fb, k0, lds_frag, kp);
   }
 }
 
@@ -387,46 +465,75 @@ __global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, int nch
 
 // ---------------------------------------------------------------- host side
 struct MsmPlan {
-  int ET, FBK, nfb, nfpad, chunks, s_per_chunk;
+  int ET, FBK, nfb, nfpad, chunks, s_per_chunk, threads;
   size_t lds, kpart_bytes;
 };
 
-static MsmPlan msm_plan(const Problem& p) {
+static int device_cus() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    return 256;
+  return cus;
+}
+
+// Scatterer chunks per frequency block: the count whose last round of
+// workgroups is fullest (rounds / chunks smallest), at least 8 tiles per chunk.
+static int pick_chunks(int nfb, int tiles, int slots) {
+  const int cmax = std::max(1, std::min(16, tiles / 8));
+  double best = 1e30;
+  int best_c = 1;
+  for (int c = 1; c <= cmax; ++c) {
+    const double rounds = (double)(((long long)nfb * c + slots - 1) / slots);
+    const double cost = rounds / c;
+    if (cost < best * 0.995) {
+      best = cost;
+      best_c = c;
+    }
+  }
+  return best_c;
+}
+
+template <int ET>
+static MsmPlan msm_plan_et(const Problem& p) {
+  using SH = MsmShape<ET>;
   MsmPlan m{};
-  m.ET = p.E <= 64 ? 4 : 8;
-  m.FBK = 2;
+  m.ET = ET;
+  m.FBK = SH::FBK;
+  m.threads = SH::THREADS;
   m.nfb = (p.NF + m.FBK - 1) / m.FBK;
   m.nfpad = m.nfb * m.FBK;
-  const int tiles = (p.S + MST - 1) / MST;
-  int chunks = std::max(1, (2048 + m.nfb - 1) / m.nfb);
-  chunks = std::min(chunks, std::max(1, tiles / 8));  // keep >= 8 tiles per chunk
-  m.s_per_chunk = std::max(1, (tiles + chunks - 1) / chunks) * MST;
+  const int tiles = std::max(1, (p.S + MST - 1) / MST);
+  const int chunks = pick_chunks(m.nfb, tiles, device_cus() * SH::WG_PER_CU);
+  m.s_per_chunk = ((tiles + chunks - 1) / chunks) * MST;
   m.chunks = std::max(1, (p.S + m.s_per_chunk - 1) / m.s_per_chunk);
-  const size_t ep = (size_t)m.ET * 16;
+  const size_t ep = (size_t)ET * 16;
   m.kpart_bytes = (size_t)m.chunks * m.nfpad * ep * ep * sizeof(cf);
-  m.lds = (size_t)m.FBK * 2 * 2 * m.ET * 64 * 16;
+  m.lds = FragLds<ET, SH::FBK>::bytes;
   return m;
 }
+
+static MsmPlan msm_plan(const Problem& p) { return p.E <= 64 ? msm_plan_et<4>(p) : msm_plan_et<8>(p); }
 
 size_t mfma_workspace(const Problem& p) {
   if (p.E > 128 || p.NS > 4) return 256;
   return (msm_plan(p).kpart_bytes + 255) / 256 * 256;
 }
 
-template <int NSUB, int ET, int FBK, bool FD>
+template <int NSUB, int ET, bool FD>
 static int launch_msm(const Problem& p, const MsmPlan& m, cf* kpart, hipStream_t st) {
-  auto kern = k_msm<NSUB, ET, FBK, FD>;
+  auto kern = k_msm<NSUB, ET, FD>;
   if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds) != hipSuccess)
     return FS_ERR_RESOURCE;
   const int slot = timing_start(st);
-  hipLaunchKernelGGL(kern, dim3(m.nfb, m.chunks), dim3(256), m.lds, st, p, m.s_per_chunk, m.nfpad, kpart);
+  hipLaunchKernelGGL(kern, dim3(m.nfb, m.chunks), dim3(m.threads), m.lds, st, p, m.s_per_chunk, m.nfpad, kpart);
   timing_stop(slot, st);
   return FS_OK;
 }
 
 template <int NSUB, int ET>
 static int launch_msm_fd(const Problem& p, const MsmPlan& m, cf* kpart, hipStream_t st) {
-  return p.fulldir ? launch_msm<NSUB, ET, 2, true>(p, m, kpart, st) : launch_msm<NSUB, ET, 2, false>(p, m, kpart, st);
+  return p.fulldir ? launch_msm<NSUB, ET, true>(p, m, kpart, st) : launch_msm<NSUB, ET, false>(p, m, kpart, st);
 }
 
 template <int NSUB>
