@@ -16,22 +16,26 @@
 // Complex -> real: the 32-deep MFMA contraction index runs over
 // (real part, imaginary part) of 16 scatterers; one fragment of element e
 // holds [Re H(e, s0..s3), Im H(e, s0..s3)] for the lane group's 4 scatterers.
-// With that fragment F(e) as both operands:
-//   Re K = (Re H | -Im H)(e) . sign (Re H | Im H)(e')
-//   Im K = (Im H |  Re H)(e) . sign (Re H | Im H)(e')
-// i.e. the A operands are the fragment with its upper half negated or its
-// halves swapped, and the B operand is the fragment with the sign of rc
-// applied: no transposes, one LDS layout for both sides.
+// With that fragment F(e) on both sides:
+//   Re K = sign (Re H | -Im H)(e) . (Re H | Im H)(e')
+//   Im K = sign (Im H |  Re H)(e) . (Re H | Im H)(e')
+// i.e. the A operands are the fragment with the sign of rc applied and its
+// upper half negated or its halves swapped (bit operations on the loaded
+// registers), and the B operand is the fragment as stored: one LDS layout for
+// both sides, no transposes.
 //
-// Workgroup = 4 waves, one block of FBK frequencies and one chunk of
-// scatterers. Per tile of 16 scatterers:
-//   1. every thread evaluates G for 4 scatterers x EPT elements at the FBK
-//      frequencies (FP64 r and phase seed, FP32 recurrence inside the block)
-//      and writes split-bf16 fragments to LDS;
+// Workgroup = ET waves (one thread per scatterer quad and element), one block
+// of FBK frequencies and one chunk of scatterers. Per tile of 16 scatterers:
+//   1. every thread evaluates G for 4 scatterers x 1 element at the FBK
+//      frequencies (FP64 distance and phase seed once per block, FP32
+//      recurrence inside it) and writes split-bf16 fragments to LDS;
 //   2. every wave runs its share of the upper-triangular 16x16 tiles of K_f
-//      (2x2 block split of the tile grid) with accumulators in registers.
-// Partial K per scatterer chunk goes to HBM; k_msm_apply sums the chunks in a
-// fixed order and applies every transmit (E x E x n_tx per frequency, FP32).
+//      for its frequencies, with accumulators in registers.
+// The LDS holds two tiles: tile n is multiplied while tile n + 1 is generated,
+// and the two waves of a SIMD take these in opposite order so that matrix-core
+// and vector work overlap. Partial K per scatterer chunk goes to HBM;
+// k_msm_apply sums the chunks in a fixed order and applies every transmit
+// (E x E x n_tx per frequency, FP32).
 #include <algorithm>
 
 #include "fs_device.h"
@@ -46,12 +50,16 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int MST = 16;  // scatterers per tile (32-deep contraction: re + im)
 
+// LDS of one workgroup: two buffers (the tile being multiplied and the tile
+// being generated) of split-bf16 fragments [k][hi|lo][tile][lane], then the
+// sign masks of rc per buffer and scatterer quad (m01, m23).
 template <int ET, int FBK>
 struct FragLds {
-  static constexpr int per_k = 2 * 2 * ET * 64;  // (T1|T2) x (hi|lo) x tiles x lanes
-  static constexpr size_t bytes = (size_t)FBK * per_k * 16;
-  __device__ static __forceinline__ int idx(int k, int t, int hl, int tile, int lane) {
-    return (((k * 2 + t) * 2 + hl) * ET + tile) * 64 + lane;
+  static constexpr int per_buf = FBK * 2 * ET * 64;  // u32x4 entries per buffer
+  static constexpr size_t frag_bytes = (size_t)2 * per_buf * 16;
+  static constexpr size_t bytes = frag_bytes + 2 * 4 * sizeof(uint2);
+  __device__ static __forceinline__ int idx(int k, int hl, int tile, int lane) {
+    return ((k * 2 + hl) * ET + tile) * 64 + lane;
   }
 };
 
@@ -121,15 +129,24 @@ __host__ __device__ constexpr bool uses_col(int c) {
 }
 
 // Workgroup shape: one thread per (scatterer quad, element) of a tile, i.e.
-// ET waves; FBK = 4 frequencies per block, two per wave, WPP waves per pair.
-template <int ET>
+// ET waves; FBK frequencies per block, KPW per wave, WPP waves per group of KPW
+// frequencies. HEAVY variants (more than two sub-elements, or directivity at
+// every frequency) need more registers for the Green's functions and take
+// FBK = 2, KPW = 1 (half the accumulators); the others FBK = 4, KPW = 2.
+template <int ET, bool HEAVY>
 struct MsmShape {
-  static constexpr int FBK = 4;
+  static constexpr int FBK = HEAVY ? 2 : 4;
+  static constexpr int KPW = FBK / 2;           // frequencies per wave
   static constexpr int NW = ET;                 // waves per workgroup
   static constexpr int THREADS = 64 * NW;
-  static constexpr int WPP = NW * 2 / FBK;      // waves sharing one frequency pair
+  static constexpr int WPP = NW * KPW / FBK;    // waves sharing one frequency group
   static constexpr int WG_PER_CU = ET == 8 ? 1 : 2;
 };
+
+// Green's-function variants of k_msm: FAST = soft baffle, centre-frequency
+// directivity with |kappa sin(theta)| <= pi (branch-free); GENERIC = any
+// baffle, centre-frequency directivity; FULLDIR = directivity at every frequency.
+enum MsmMode { MSM_FAST = 0, MSM_GENERIC = 1, MSM_FULLDIR = 2 };
 
 // sin(a)/a for |a| <= pi: Taylor series to a^16 (truncation < 1e-8 on that range).
 __device__ __forceinline__ float sinc_poly_pi(float a) {
@@ -149,6 +166,7 @@ template <int NSUB>
 struct ElemData {
   double sx[NSUB], sz[NSUB];
   float ce, se;
+  int ee;  // element index clamped into range
   bool valid;
 };
 
@@ -158,6 +176,7 @@ __device__ __forceinline__ ElemData<NSUB> load_element(const Problem& p) {
   const int e = threadIdx.x % (ET * 16);
   d.valid = e < p.E;
   const int ee = d.valid ? e : 0;
+  d.ee = ee;
 #pragma unroll
   for (int j = 0; j < NSUB; ++j) {
     d.sx[j] = p.sub_x[ee * NSUB + j];
@@ -168,32 +187,54 @@ __device__ __forceinline__ ElemData<NSUB> load_element(const Problem& p) {
   return d;
 }
 
+// Frequency-block constants: phase and attenuation per metre of path.
+struct BlockScalars {
+  double nu0;  // f_b / c      (revolutions per metre at the block's first frequency)
+  float nud;   // df / c       (revolutions per metre per frequency step)
+  float a0;    // att_coef log2(e) f_b
+  float ad;    // att_coef log2(e) df
+  float fb, df;
+  bool poly_ok;
+};
+
+__device__ __forceinline__ BlockScalars block_scalars(const Problem& p, double fb_hz) {
+  BlockScalars b;
+  b.nu0 = fb_hz * p.inv_c;
+  b.nud = (float)(p.df * p.inv_c);
+  b.a0 = (float)((double)p.att_coef * (double)kLog2e * fb_hz);
+  b.ad = (float)((double)p.att_coef * (double)kLog2e * p.df);
+  b.fb = (float)fb_hz;
+  b.df = (float)p.df;
+  b.poly_ok = p.kappa <= kPi;
+  return b;
+}
+
 // ---------------------------------------------------------------- G fragments
 // Thread (scatterer quad g, element e): for each of the quad's 4 scatterers
 // and each sub-element, the frequency-independent geometry once per block
 // (FP64 distance and phase reduction), the seed g(f_b) and the step
 // q = exp((-att + i 2 pi t) df), then the block's FBK frequencies by the FP32
 // recurrence g <- g q. Sub-elements are summed, sqrt|rc| is folded into the
-// amplitude, and every frequency is written as split-bf16 fragments
-// (T1 plain, T2 with the sign of rc applied).
-template <int NSUB, int FBK, bool FULLDIR, int ET>
-__device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<NSUB>& ed, bool poly_ok, int s0,
-                                              int s_end, double fb_hz, u32x4* lds) {
+// amplitude and every frequency is written as split-bf16 fragments; the sign
+// of rc goes to the quad's mask.
+template <int NSUB, int FBK, int MODE, int ET>
+__device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<NSUB>& ed, const BlockScalars& bs,
+                                              int s0, int s_end, u32x4* frag, uint2* masks) {
   using L = FragLds<ET, FBK>;
   const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / (ET * 16)));  // wave-uniform
   const int e = threadIdx.x % (ET * 16);
-  cf gk[4][NSUB], q[4][NSUB];
-  float sth[4][NSUB];
+  float re[FBK][4], im[FBK][4];
   unsigned m01 = 0u, m23 = 0u;
-  const float fbf = (float)fb_hz, dff = (float)p.df;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int k = 0; k < FBK; ++k) re[k][i] = im[k][i] = 0.f;
     const int s = s0 + 4 * g + i;
     const bool valid = s < s_end;
     const int sc = valid ? s : s_end - 1;  // in-bounds load, masked below
     const float x = p.x[sc], z = p.z[sc], r = p.rc[sc];
-    const bool live = valid && ed.valid && r != 0.f && !out_of_field(p, x, z);
-    const float w = live ? __builtin_amdgcn_sqrtf(fabsf(r)) * p.inv_nsub : 0.f;
+    const bool live = valid && r != 0.f && !out_of_field(p, x, z);
+    const float w = (live && ed.valid) ? __builtin_amdgcn_sqrtf(fabsf(r)) * p.inv_nsub : 0.f;
     const unsigned sb = (live && r < 0.f) ? 1u : 0u;
     if (i < 2)
       m01 |= sb << (15 + 16 * i);
@@ -202,8 +243,12 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<N
     const double xd = (double)x, zd = (double)z;
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
-      const double dx = xd - ed.sx[j];
-      const double dz = zd - ed.sz[j];
+      // sub-element centroids: held in registers where they fit, else re-read (cached)
+      constexpr bool held = NSUB <= 2 && MODE != MSM_FULLDIR;
+      const double sxj = held ? ed.sx[j] : p.sub_x[ed.ee * NSUB + j];
+      const double szj = held ? ed.sz[j] : p.sub_z[ed.ee * NSUB + j];
+      const double dx = xd - sxj;
+      const double dz = zd - szj;
       const double r2 = fma(dx, dx, dz * dz);
       const bool zero = !(r2 > 0.0);
       double y = __builtin_amdgcn_rsq(zero ? 1.0 : r2);
@@ -214,98 +259,101 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<N
       const float ca = zero ? 0.0f : fabsf((float)dz) * rinv;
       const float st = sa * ed.ce - ca * ed.se;
       const float ct = ca * ed.ce + sa * ed.se;
-      float obl = (p.baffle == FS_BAFFLE_SOFT)    ? ct
-                  : (p.baffle == FS_BAFFLE_RIGID) ? 1.0f
-                                                  : ct * __builtin_amdgcn_rcpf(ct + p.baffle_b);
+      float obl;
+      if constexpr (MODE == MSM_FAST) {
+        obl = ct;
+      } else {
+        obl = (p.baffle == FS_BAFFLE_SOFT)    ? ct
+              : (p.baffle == FS_BAFFLE_RIGID) ? 1.0f
+                                              : ct * __builtin_amdgcn_rcpf(ct + p.baffle_b);
+      }
       if (!(ct > 0.0f)) obl = 1e-16f;
       const float rcf = (float)rcl;
       float amp = obl * __builtin_amdgcn_rsqf(rcf) * w;
-      if constexpr (!FULLDIR) amp *= poly_ok ? sinc_poly_pi(p.kappa * st) : sinc_fast(p.kappa * st);
-      sth[i][j] = st;
-      const double t = rcl * p.inv_c;
-      const float att = p.att_coef * rcf * kLog2e;
-      const double nu0 = fb_hz * t, nud = p.df * t;
-      const float ph0 = (float)(nu0 - floor(nu0)), phd = (float)(nud - floor(nud));
-      const float m0 = amp * __builtin_amdgcn_exp2f(-att * fbf);
-      const float md = __builtin_amdgcn_exp2f(-att * dff);
-      gk[i][j] = {m0 * __builtin_amdgcn_cosf(ph0), m0 * __builtin_amdgcn_sinf(ph0)};
-      q[i][j] = {md * __builtin_amdgcn_cosf(phd), md * __builtin_amdgcn_sinf(phd)};
-    }
-  }
-  const int tile = e >> 4;
-  const int lane = (e & 15) + 16 * g;
-  const u32x4 msk = {m01, m23, m01, m23};
+      if constexpr (MODE == MSM_FAST) {
+        amp *= sinc_poly_pi(p.kappa * st);
+      } else if constexpr (MODE == MSM_GENERIC) {
+        amp *= bs.poly_ok ? sinc_poly_pi(p.kappa * st) : sinc_fast(p.kappa * st);
+      }
+      const double nu0 = bs.nu0 * rcl;
+      const float ph0 = (float)(nu0 - floor(nu0));
+      const float phd = __builtin_amdgcn_fractf(bs.nud * rcf);
+      const float m0 = amp * __builtin_amdgcn_exp2f(-bs.a0 * rcf);
+      const float md = __builtin_amdgcn_exp2f(-bs.ad * rcf);
+      cf gv = {m0 * __builtin_amdgcn_cosf(ph0), m0 * __builtin_amdgcn_sinf(ph0)};
+      const cf q = {md * __builtin_amdgcn_cosf(phd), md * __builtin_amdgcn_sinf(phd)};
 #pragma unroll
-  for (int k = 0; k < FBK; ++k) {
-    float re[4], im[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      re[i] = im[i] = 0.f;
-#pragma unroll
-      for (int j = 0; j < NSUB; ++j) {
-        if constexpr (FULLDIR) {
-          const float d = sinc_fast(p.kappa_per_hz * (fbf + (float)k * dff) * sth[i][j]);
-          re[i] = fmaf(d, gk[i][j].x, re[i]);
-          im[i] = fmaf(d, gk[i][j].y, im[i]);
+      for (int k = 0; k < FBK; ++k) {
+        if constexpr (MODE == MSM_FULLDIR) {
+          const float d = sinc_fast(p.kappa_per_hz * fmaf((float)k, bs.df, bs.fb) * st);
+          re[k][i] = fmaf(d, gv.x, re[k][i]);
+          im[k][i] = fmaf(d, gv.y, im[k][i]);
         } else {
-          re[i] += gk[i][j].x;
-          im[i] += gk[i][j].y;
+          re[k][i] += gv.x;
+          im[k][i] += gv.y;
         }
-        if (k + 1 < FBK) gk[i][j] = cmul(gk[i][j], q[i][j]);
+        if (k + 1 < FBK) gv = cmul(gv, q);
       }
     }
+  }
+  if (e == 0) masks[g] = make_uint2(m01, m23);
+  const int tile = e >> 4;
+  const int lane = (e & 15) + 16 * g;
+#pragma unroll
+  for (int k = 0; k < FBK; ++k) {
     unsigned h0, h1, h2, h3, l0, l1, l2, l3;
-    split_pair(re[0], re[1], h0, l0);
-    split_pair(re[2], re[3], h1, l1);
-    split_pair(im[0], im[1], h2, l2);
-    split_pair(im[2], im[3], h3, l3);
-    const u32x4 hi = {h0, h1, h2, h3}, lo = {l0, l1, l2, l3};
-    lds[L::idx(k, 0, 0, tile, lane)] = hi;
-    lds[L::idx(k, 0, 1, tile, lane)] = lo;
-    lds[L::idx(k, 1, 0, tile, lane)] = hi ^ msk;
-    lds[L::idx(k, 1, 1, tile, lane)] = lo ^ msk;
+    split_pair(re[k][0], re[k][1], h0, l0);
+    split_pair(re[k][2], re[k][3], h1, l1);
+    split_pair(im[k][0], im[k][1], h2, l2);
+    split_pair(im[k][2], im[k][3], h3, l3);
+    frag[L::idx(k, 0, tile, lane)] = (u32x4){h0, h1, h2, h3};
+    frag[L::idx(k, 1, tile, lane)] = (u32x4){l0, l1, l2, l3};
   }
 }
 
 // ---------------------------------------------------------------- MFMA tiles of one wave
-// Accumulators of one wave: its job list at the two frequencies k0, k0 + 1 of the block.
-template <int ET, int FBK, int W>
+// Accumulators of one wave: its job list at the KPW frequencies k0 .. k0 + KPW - 1 of the block.
+template <int ET, int FBK, int KPW, int W>
 struct WaveTiles {
   using J = JobList<ET, W>;
   static constexpr int NJ = J::n;
-  floatx4 acc[2][NJ][2];
+  floatx4 acc[KPW][NJ][2];
 
   __device__ __forceinline__ void zero() {
 #pragma unroll
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < KPW; ++k)
 #pragma unroll
       for (int j = 0; j < NJ; ++j) acc[k][j][0] = acc[k][j][1] = (floatx4){0.f, 0.f, 0.f, 0.f};
   }
 
-  // One 16-scatterer tile: for each of the wave's frequencies, load the sign-applied
-  // column fragments this wave needs, then stream its rows.
-  __device__ __forceinline__ void step(const u32x4* lds, int k0) {
+  // One 16-scatterer tile: for each of the wave's frequencies, load the column
+  // fragments this wave needs, then stream its rows. The row (A) operands carry
+  // the sign of rc of the lane's scatterer quad: Re K = sign (Re | -Im),
+  // Im K = sign (Im | Re), both against (Re | Im) of the column.
+  __device__ __forceinline__ void step(const u32x4* frag, const uint2* masks, int k0) {
     using L = FragLds<ET, FBK>;
     const int lane = threadIdx.x & 63;
-    const u32x4 nim = {0u, 0u, 0x80008000u, 0x80008000u};
+    const uint2 m = masks[lane >> 4];
+    const u32x4 mr = {m.x, m.y, m.x ^ 0x80008000u, m.y ^ 0x80008000u};
+    const u32x4 mi = {m.x, m.y, m.x, m.y};
 #pragma unroll
-    for (int kk = 0; kk < 2; ++kk) {
+    for (int kk = 0; kk < KPW; ++kk) {
       const int k = k0 + kk;
       u32x4 bh[ET], bl[ET];
 #pragma unroll
       for (int c = 0; c < ET; ++c) {
         if (uses_col<ET, W>(c)) {
-          bh[c] = lds[L::idx(k, 1, 0, c, lane)];
-          bl[c] = lds[L::idx(k, 1, 1, c, lane)];
+          bh[c] = frag[L::idx(k, 0, c, lane)];
+          bl[c] = frag[L::idx(k, 1, c, lane)];
         }
       }
 #pragma unroll
       for (int r = 0; r < ET; ++r) {
         if (!uses_row<ET, W>(r)) continue;
-        const u32x4 h = lds[L::idx(k, 0, 0, r, lane)];
-        const u32x4 l = lds[L::idx(k, 0, 1, r, lane)];
-        const u32x4 arh = h ^ nim, arl = l ^ nim;
-        const u32x4 aih = {h.z, h.w, h.x, h.y}, ail = {l.z, l.w, l.x, l.y};
+        const u32x4 h = frag[L::idx(k, 0, r, lane)];
+        const u32x4 l = frag[L::idx(k, 1, r, lane)];
+        const u32x4 arh = h ^ mr, arl = l ^ mr;
+        const u32x4 aih = (u32x4){h.z, h.w, h.x, h.y} ^ mi, ail = (u32x4){l.z, l.w, l.x, l.y} ^ mi;
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
           if (J::a[j] != r) continue;
@@ -328,7 +376,7 @@ struct WaveTiles {
   __device__ __forceinline__ void store(cf* kpart, int E_pad, int kglob0) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int kk = 0; kk < 2; ++kk) {
+    for (int kk = 0; kk < KPW; ++kk) {
       cf* Kk = kpart + (size_t)(kglob0 + kk) * E_pad * E_pad;
 #pragma unroll
       for (int j = 0; j < NJ; ++j) {
@@ -343,49 +391,66 @@ struct WaveTiles {
   }
 };
 
-template <int NSUB, int ET, bool FULLDIR, int W>
-__device__ __forceinline__ void msm_wave(const Problem& p, int sb, int se, int fb, int k0, u32x4* lds, cf* kpart) {
-  constexpr int FBK = MsmShape<ET>::FBK;
+// One wave's part of a workgroup: tiles are generated one ahead into the other
+// LDS buffer. The two waves that share a SIMD run in opposite order within an
+// iteration (one multiplies while the other generates) so that matrix-core and
+// vector work overlap; one barrier per tile separates the buffers' roles.
+template <int NSUB, int ET, int MODE, int W>
+__device__ __forceinline__ void msm_wave(const Problem& p, int sb, int se, int fb, int k0, bool mfma_first,
+                                         char* lds, cf* kpart) {
+  using SH = MsmShape<ET, (NSUB > 2 || MODE == MSM_FULLDIR)>;
+  constexpr int FBK = SH::FBK;
+  using L = FragLds<ET, FBK>;
+  u32x4* const frag = (u32x4*)lds;                    // buffer b at frag + b * per_buf
+  uint2* const masks = (uint2*)(lds + L::frag_bytes);  // buffer b at masks + 4 b
   const ElemData<NSUB> ed = load_element<NSUB, ET>(p);
-  const bool poly_ok = !FULLDIR && p.kappa <= kPi;  // centre-frequency sinc argument |kappa sin| <= pi
-  const double fb_hz = p.f0 + (double)(fb * FBK) * p.df;
-  WaveTiles<ET, FBK, W> t;
+  const BlockScalars bs = block_scalars(p, p.f0 + (double)(fb * FBK) * p.df);
+  WaveTiles<ET, FBK, SH::KPW, W> t;
   t.zero();
+  int buf = 0;
+  if (sb < se) gen_fragments<NSUB, FBK, MODE, ET>(p, ed, bs, sb, se, frag, masks);
+  __syncthreads();
   for (int s0 = sb; s0 < se; s0 += MST) {
-    gen_fragments<NSUB, FBK, FULLDIR, ET>(p, ed, poly_ok, s0, se, fb_hz, lds);
+    const bool more = s0 + MST < se;
+    const int nb = buf ^ 1;
+    if (mfma_first) t.step(frag + buf * L::per_buf, masks + 4 * buf, k0);
+    if (more) gen_fragments<NSUB, FBK, MODE, ET>(p, ed, bs, s0 + MST, se, frag + nb * L::per_buf, masks + 4 * nb);
+    if (!mfma_first) t.step(frag + buf * L::per_buf, masks + 4 * buf, k0);
     __syncthreads();
-    t.step(lds, k0);
-    __syncthreads();
+    buf = nb;
   }
   t.store(kpart, ET * 16, fb * FBK + k0);
 }
 
 // Grid (frequency block, scatterer chunk); ET waves per workgroup. Wave w owns
-// frequencies 2 (w / WPP) and 2 (w / WPP) + 1 of the block and job list w % WPP.
-template <int NSUB, int ET, bool FULLDIR>
-__global__ __launch_bounds__(MsmShape<ET>::THREADS, MsmShape<ET>::WG_PER_CU) void k_msm(Problem p, int s_per_chunk,
-                                                                                         int nfpad, cf* __restrict__ kpart) {
-  using SH = MsmShape<ET>;
-  extern __shared__ u32x4 lds_frag[];
+// frequencies KPW (w / WPP) .. KPW (w / WPP) + KPW - 1 of the block and job list
+// w % WPP. Waves w and w + 4 (and w, w + 1) take opposite orders of multiply and
+// generate.
+template <int NSUB, int ET, int MODE>
+__global__ __launch_bounds__(64 * ET, ET == 8 ? 1 : 2) void k_msm(Problem p, int s_per_chunk, int nfpad,
+                                                                 cf* __restrict__ kpart) {
+  using SH = MsmShape<ET, (NSUB > 2 || MODE == MSM_FULLDIR)>;
+  extern __shared__ __attribute__((aligned(16))) char lds_raw[];
   const int fb = blockIdx.x;
   const int chunk = blockIdx.y;
   const int sb = chunk * s_per_chunk;
   const int se = min(sb + s_per_chunk, p.S);
   cf* kp = kpart + (size_t)chunk * nfpad * (ET * 16) * (ET * 16);
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int k0 = 2 * (w / SH::WPP);
+  const int k0 = SH::KPW * (w / SH::WPP);
+  const bool mfma_first = ((w ^ (w >> 2)) & 1) == 0;
   if constexpr (SH::WPP == 4) {
     switch (w % 4) {
-      case 0: msm_wave<NSUB, ET, FULLDIR, 0>(p, sb, se, fb, k0, lds_frag, kp); break;
-      case 1: msm_wave<NSUB, ET, FULLDIR, 1>(p, sb, se, fb, k0, lds_frag, kp); break;
-      case 2: msm_wave<NSUB, ET, FULLDIR, 2>(p, sb, se, fb, k0, lds_frag, kp); break;
-      default: msm_wave<NSUB, ET, FULLDIR, 3>(p, sb, se, fb, k0, lds_frag, kp); break;
+      case 0: msm_wave<NSUB, ET, MODE, 0>(p, sb, se, fb, k0, mfma_first, lds_raw, kp); break;
+      case 1: msm_wave<NSUB, ET, MODE, 1>(p, sb, se, fb, k0, mfma_first, lds_raw, kp); break;
+      case 2: msm_wave<NSUB, ET, MODE, 2>(p, sb, se, fb, k0, mfma_first, lds_raw, kp); break;
+      default: msm_wave<NSUB, ET, MODE, 3>(p, sb, se, fb, k0, mfma_first, lds_raw, kp); break;
     }
   } else {
     if (w % 2 == 0)
-      msm_wave<NSUB, ET, FULLDIR, 0>(p, sb, se, fb, k0, lds_frag, kp);
+      msm_wave<NSUB, ET, MODE, 0>(p, sb, se, fb, k0, mfma_first, lds_raw, kp);
     else
-      msm_wave<NSUB, ET, FULLDIR, 1>(p, sb, se, fb, k0, lds_frag, kp);
+      msm_wave<NSUB, ET, MODE, 1>(p, sb, se, fb, k0, mfma_first, lds_raw, kp);
   }
 }
 
@@ -494,9 +559,9 @@ static int pick_chunks(int nfb, int tiles, int slots) {
   return best_c;
 }
 
-template <int ET>
+template <int ET, bool HEAVY>
 static MsmPlan msm_plan_et(const Problem& p) {
-  using SH = MsmShape<ET>;
+  using SH = MsmShape<ET, HEAVY>;
   MsmPlan m{};
   m.ET = ET;
   m.FBK = SH::FBK;
@@ -513,16 +578,20 @@ static MsmPlan msm_plan_et(const Problem& p) {
   return m;
 }
 
-static MsmPlan msm_plan(const Problem& p) { return p.E <= 64 ? msm_plan_et<4>(p) : msm_plan_et<8>(p); }
+static MsmPlan msm_plan(const Problem& p) {
+  const bool heavy = p.NS > 2 || p.fulldir;
+  if (p.E <= 64) return heavy ? msm_plan_et<4, true>(p) : msm_plan_et<4, false>(p);
+  return heavy ? msm_plan_et<8, true>(p) : msm_plan_et<8, false>(p);
+}
 
 size_t mfma_workspace(const Problem& p) {
   if (p.E > 128 || p.NS > 4) return 256;
   return (msm_plan(p).kpart_bytes + 255) / 256 * 256;
 }
 
-template <int NSUB, int ET, bool FD>
+template <int NSUB, int ET, int MODE>
 static int launch_msm(const Problem& p, const MsmPlan& m, cf* kpart, hipStream_t st) {
-  auto kern = k_msm<NSUB, ET, FD>;
+  auto kern = k_msm<NSUB, ET, MODE>;
   if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds) != hipSuccess)
     return FS_ERR_RESOURCE;
   const int slot = timing_start(st);
@@ -533,7 +602,9 @@ static int launch_msm(const Problem& p, const MsmPlan& m, cf* kpart, hipStream_t
 
 template <int NSUB, int ET>
 static int launch_msm_fd(const Problem& p, const MsmPlan& m, cf* kpart, hipStream_t st) {
-  return p.fulldir ? launch_msm<NSUB, ET, true>(p, m, kpart, st) : launch_msm<NSUB, ET, false>(p, m, kpart, st);
+  if (p.fulldir) return launch_msm<NSUB, ET, MSM_FULLDIR>(p, m, kpart, st);
+  if (p.baffle == FS_BAFFLE_SOFT && p.kappa <= kPi) return launch_msm<NSUB, ET, MSM_FAST>(p, m, kpart, st);
+  return launch_msm<NSUB, ET, MSM_GENERIC>(p, m, kpart, st);
 }
 
 template <int NSUB>

@@ -57,6 +57,10 @@ CASES = {
                                       {"full_frequency_directivity": True}),
     "nan_delays": lambda: (fs.L11_5v(), fs.MediumParams(), *speckle(6, 20),
                            np.where(np.arange(128) % 17 == 3, np.nan, 0.0), {}),
+    "p4_nsub3_fulldir": lambda: (fs.P4_2v().model_copy(update={"baffle": "rigid"}), fs.MediumParams(attenuation=0.2),
+                                 *speckle(7, 37, zlim=(0.01, 0.08)),
+                                 fs.focused(_pos(fs.P4_2v()), np.array([0.005, 0.04]), speed_of_sound=1540.0),
+                                 {"element_splitting": 3, "full_frequency_directivity": True}),
 }
 
 
@@ -69,8 +73,7 @@ def test_fp32_matches_oracle(name, cuda):
     assert rel_l2(rf, want) < FP32_TOL
 
 
-@pytest.mark.parametrize("name", ["l11_speckle_atten", "convex_c5", "impedance_nsub3", "p4_focused_ref",
-                                  "full_freq_directivity"])
+@pytest.mark.parametrize("name", list(CASES))
 def test_mfma_matches_oracle(name, cuda):
     params, medium, x, z, rc, D, kw = CASES[name]()
     print(name, flush=True)
