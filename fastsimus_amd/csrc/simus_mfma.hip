@@ -9,20 +9,21 @@
 //   K_f[e, e'] = sum_{s in field} rc_s G(f, e, s) G(f, e', s)        (complex symmetric, E x E)
 //
 // so the scatterer sum is done once for all transmits. K_f is a rank-S
-// product H^T diag(sign rc) H with H = sqrt|rc| G, evaluated with
+// product H^T H with H = sqrt(rc) G (see below for rc < 0), evaluated with
 // v_mfma_f32_16x16x32_bf16 in split precision: every operand is hi + lo
 // bf16 and hi*hi + hi*lo + lo*hi is accumulated in FP32.
 //
 // Complex -> real: the 32-deep MFMA contraction index runs over
 // (real part, imaginary part) of 16 scatterers; one fragment of element e
 // holds [Re H(e, s0..s3), Im H(e, s0..s3)] for the lane group's 4 scatterers.
-// With that fragment F(e) on both sides:
-//   Re K = sign (Re H | -Im H)(e) . (Re H | Im H)(e')
-//   Im K = sign (Im H |  Re H)(e) . (Re H | Im H)(e')
-// i.e. the A operands are the fragment with the sign of rc applied and its
-// upper half negated or its halves swapped (bit operations on the loaded
-// registers), and the B operand is the fragment as stored: one LDS layout for
-// both sides, no transposes.
+// The sign of rc is carried by H itself: H = i sqrt|rc| G for rc < 0 (a
+// quarter revolution added to the phase), so that H H^T = rc G G^T. With the
+// fragment F(e) on both sides:
+//   Re K = (Re H | -Im H)(e) . (Re H | Im H)(e')
+//   Im K = (Im H |  Re H)(e) . (Re H | Im H)(e')
+// i.e. the A operands are the fragment with its upper half negated or its
+// halves swapped, and the B operand is the fragment as stored: one LDS layout
+// for both sides, no transposes.
 //
 // Workgroup = ET waves (one thread per scatterer quad and element), one block
 // of FBK frequencies and one chunk of scatterers. Per tile of 16 scatterers:
@@ -51,13 +52,11 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 constexpr int MST = 16;  // scatterers per tile (32-deep contraction: re + im)
 
 // LDS of one workgroup: two buffers (the tile being multiplied and the tile
-// being generated) of split-bf16 fragments [k][hi|lo][tile][lane], then the
-// sign masks of rc per buffer and scatterer quad (m01, m23).
+// being generated) of split-bf16 fragments [k][hi|lo][tile][lane].
 template <int ET, int FBK>
 struct FragLds {
   static constexpr int per_buf = FBK * 2 * ET * 64;  // u32x4 entries per buffer
-  static constexpr size_t frag_bytes = (size_t)2 * per_buf * 16;
-  static constexpr size_t bytes = frag_bytes + 2 * 4 * sizeof(uint2);
+  static constexpr size_t bytes = (size_t)2 * per_buf * 16;
   __device__ static __forceinline__ int idx(int k, int hl, int tile, int lane) {
     return ((k * 2 + hl) * ET + tile) * 64 + lane;
   }
@@ -78,41 +77,41 @@ __device__ __forceinline__ floatx4 mma(u32x4 a, u32x4 b, floatx4 c) {
 }
 
 // ---------------------------------------------------------------- tile jobs
-// Upper-triangular 16x16 tile pairs (a <= b) of the E x E matrix. A frequency
-// pair of the block is shared by WPP waves, each owning one list: 9 pairs
-// each for 8 tiles (E <= 128, WPP = 4), 5 each for 4 tiles (E <= 64,
-// WPP = 2), grouped so that each wave touches few distinct rows/cols.
+// Upper-triangular 16x16 tile pairs (a <= b) of the E x E matrix, a = row (A
+// operand), b = column (B operand). WPP waves share a frequency group, each
+// owning one list with only two distinct rows: 9 pairs each for 8 tiles
+// (E <= 128, WPP = 4), 5 each for 4 tiles (E <= 64, WPP = 2).
 template <int ET, int W>
 struct JobList;
 template <> struct JobList<8, 0> {
   static constexpr int n = 9;
-  static constexpr int a[9] = {0, 0, 0, 0, 1, 1, 1, 2, 2};
-  static constexpr int b[9] = {0, 1, 2, 3, 1, 2, 3, 2, 3};
+  static constexpr int a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 7};
+  static constexpr int b[9] = {0, 1, 2, 3, 4, 5, 6, 7, 7};
 };
 template <> struct JobList<8, 1> {
   static constexpr int n = 9;
-  static constexpr int a[9] = {4, 4, 4, 5, 5, 5, 6, 6, 7};
-  static constexpr int b[9] = {5, 6, 7, 5, 6, 7, 6, 7, 7};
+  static constexpr int a[9] = {1, 1, 1, 1, 1, 1, 1, 6, 6};
+  static constexpr int b[9] = {1, 2, 3, 4, 5, 6, 7, 6, 7};
 };
 template <> struct JobList<8, 2> {
   static constexpr int n = 9;
-  static constexpr int a[9] = {0, 0, 0, 0, 1, 1, 1, 1, 4};
-  static constexpr int b[9] = {4, 5, 6, 7, 4, 5, 6, 7, 4};
+  static constexpr int a[9] = {2, 2, 2, 2, 2, 2, 5, 5, 5};
+  static constexpr int b[9] = {2, 3, 4, 5, 6, 7, 5, 6, 7};
 };
 template <> struct JobList<8, 3> {
   static constexpr int n = 9;
-  static constexpr int a[9] = {2, 2, 2, 2, 3, 3, 3, 3, 3};
-  static constexpr int b[9] = {4, 5, 6, 7, 3, 4, 5, 6, 7};
+  static constexpr int a[9] = {3, 3, 3, 3, 3, 4, 4, 4, 4};
+  static constexpr int b[9] = {3, 4, 5, 6, 7, 4, 5, 6, 7};
 };
 template <> struct JobList<4, 0> {
   static constexpr int n = 5;
-  static constexpr int a[5] = {0, 0, 0, 0, 1};
-  static constexpr int b[5] = {0, 1, 2, 3, 1};
+  static constexpr int a[5] = {0, 0, 0, 0, 3};
+  static constexpr int b[5] = {0, 1, 2, 3, 3};
 };
 template <> struct JobList<4, 1> {
   static constexpr int n = 5;
-  static constexpr int a[5] = {1, 1, 2, 2, 3};
-  static constexpr int b[5] = {2, 3, 2, 3, 3};
+  static constexpr int a[5] = {1, 1, 1, 2, 2};
+  static constexpr int b[5] = {1, 2, 3, 2, 3};
 };
 
 template <int ET, int W>
@@ -209,38 +208,44 @@ __device__ __forceinline__ BlockScalars block_scalars(const Problem& p, double f
   return b;
 }
 
+// ---------------------------------------------------------------- scatterer prep
+// Per-scatterer inputs of k_msm: (x, z, w, 0) with w = sqrt|rc| / n_sub carrying
+// the sign of rc; w = 0 for rc = 0 and for scatterers out of the field.
+__global__ void k_scat_prep(Problem p, float4* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= p.S) return;
+  const float x = p.x[s], z = p.z[s], r = p.rc[s];
+  const bool live = r != 0.f && !out_of_field(p, x, z);
+  const float w = live ? copysignf(sqrtf(fabsf(r)) * p.inv_nsub, r) : 0.f;
+  out[s] = make_float4(x, z, w, 0.f);
+}
+
 // ---------------------------------------------------------------- G fragments
 // Thread (scatterer quad g, element e): for each of the quad's 4 scatterers
 // and each sub-element, the frequency-independent geometry once per block
 // (FP64 distance and phase reduction), the seed g(f_b) and the step
 // q = exp((-att + i 2 pi t) df), then the block's FBK frequencies by the FP32
 // recurrence g <- g q. Sub-elements are summed, sqrt|rc| is folded into the
-// amplitude and every frequency is written as split-bf16 fragments; the sign
-// of rc goes to the quad's mask.
+// amplitude and a quarter revolution into the phase where rc < 0 (H = i sqrt|rc| G),
+// and every frequency is written as split-bf16 fragments.
 template <int NSUB, int FBK, int MODE, int ET>
 __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<NSUB>& ed, const BlockScalars& bs,
-                                              int s0, int s_end, u32x4* frag, uint2* masks) {
+                                              const float4* __restrict__ scat, int s0, int s_end, u32x4* frag) {
   using L = FragLds<ET, FBK>;
   const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / (ET * 16)));  // wave-uniform
   const int e = threadIdx.x % (ET * 16);
   float re[FBK][4], im[FBK][4];
-  unsigned m01 = 0u, m23 = 0u;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
 #pragma unroll
     for (int k = 0; k < FBK; ++k) re[k][i] = im[k][i] = 0.f;
     const int s = s0 + 4 * g + i;
     const bool valid = s < s_end;
-    const int sc = valid ? s : s_end - 1;  // in-bounds load, masked below
-    const float x = p.x[sc], z = p.z[sc], r = p.rc[sc];
-    const bool live = valid && r != 0.f && !out_of_field(p, x, z);
-    const float w = (live && ed.valid) ? __builtin_amdgcn_sqrtf(fabsf(r)) * p.inv_nsub : 0.f;
-    const unsigned sb = (live && r < 0.f) ? 1u : 0u;
-    if (i < 2)
-      m01 |= sb << (15 + 16 * i);
-    else
-      m23 |= sb << (15 + 16 * (i - 2));
-    const double xd = (double)x, zd = (double)z;
+    const float4 sd = scat[valid ? s : s_end - 1];  // in-bounds load, masked below
+    const float ws = (valid && ed.valid) ? sd.z : 0.f;
+    const float w = fabsf(ws);
+    const double quarter = ws < 0.f ? 0.25 : 0.0;
+    const double xd = (double)sd.x, zd = (double)sd.y;
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
       // sub-element centroids: held in registers where they fit, else re-read (cached)
@@ -275,7 +280,7 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<N
       } else if constexpr (MODE == MSM_GENERIC) {
         amp *= bs.poly_ok ? sinc_poly_pi(p.kappa * st) : sinc_fast(p.kappa * st);
       }
-      const double nu0 = bs.nu0 * rcl;
+      const double nu0 = fma(bs.nu0, rcl, quarter);
       const float ph0 = (float)(nu0 - floor(nu0));
       const float phd = __builtin_amdgcn_fractf(bs.nud * rcf);
       const float m0 = amp * __builtin_amdgcn_exp2f(-bs.a0 * rcf);
@@ -296,7 +301,6 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<N
       }
     }
   }
-  if (e == 0) masks[g] = make_uint2(m01, m23);
   const int tile = e >> 4;
   const int lane = (e & 15) + 16 * g;
 #pragma unroll
@@ -326,45 +330,43 @@ struct WaveTiles {
       for (int j = 0; j < NJ; ++j) acc[k][j][0] = acc[k][j][1] = (floatx4){0.f, 0.f, 0.f, 0.f};
   }
 
-  // One 16-scatterer tile: for each of the wave's frequencies, load the column
-  // fragments this wave needs, then stream its rows. The row (A) operands carry
-  // the sign of rc of the lane's scatterer quad: Re K = sign (Re | -Im),
-  // Im K = sign (Im | Re), both against (Re | Im) of the column.
-  __device__ __forceinline__ void step(const u32x4* frag, const uint2* masks, int k0) {
+  // One 16-scatterer tile: for each of the wave's frequencies, build the A
+  // operands of its two rows (Re K: upper half negated, Im K: halves swapped),
+  // then stream its columns.
+  __device__ __forceinline__ void step(const u32x4* frag, int k0) {
     using L = FragLds<ET, FBK>;
     const int lane = threadIdx.x & 63;
-    const uint2 m = masks[lane >> 4];
-    const u32x4 mr = {m.x, m.y, m.x ^ 0x80008000u, m.y ^ 0x80008000u};
-    const u32x4 mi = {m.x, m.y, m.x, m.y};
+    const u32x4 nim = {0u, 0u, 0x80008000u, 0x80008000u};
 #pragma unroll
     for (int kk = 0; kk < KPW; ++kk) {
       const int k = k0 + kk;
-      u32x4 bh[ET], bl[ET];
-#pragma unroll
-      for (int c = 0; c < ET; ++c) {
-        if (uses_col<ET, W>(c)) {
-          bh[c] = frag[L::idx(k, 0, c, lane)];
-          bl[c] = frag[L::idx(k, 1, c, lane)];
-        }
-      }
+      u32x4 arh[ET], aih[ET], arl[ET], ail[ET];
 #pragma unroll
       for (int r = 0; r < ET; ++r) {
         if (!uses_row<ET, W>(r)) continue;
         const u32x4 h = frag[L::idx(k, 0, r, lane)];
         const u32x4 l = frag[L::idx(k, 1, r, lane)];
-        const u32x4 arh = h ^ mr, arl = l ^ mr;
-        const u32x4 aih = (u32x4){h.z, h.w, h.x, h.y} ^ mi, ail = (u32x4){l.z, l.w, l.x, l.y} ^ mi;
+        arh[r] = h ^ nim;
+        arl[r] = l ^ nim;
+        aih[r] = (u32x4){h.z, h.w, h.x, h.y};
+        ail[r] = (u32x4){l.z, l.w, l.x, l.y};
+      }
+#pragma unroll
+      for (int c = 0; c < ET; ++c) {
+        if (!uses_col<ET, W>(c)) continue;
+        const u32x4 bh = frag[L::idx(k, 0, c, lane)];
+        const u32x4 bl = frag[L::idx(k, 1, c, lane)];
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
-          if (J::a[j] != r) continue;
-          const int c = J::b[j];
+          if (J::b[j] != c) continue;
+          const int r = J::a[j];
           floatx4 kr = acc[kk][j][0], ki = acc[kk][j][1];
-          kr = mma(arh, bh[c], kr);
-          ki = mma(aih, bh[c], ki);
-          kr = mma(arh, bl[c], kr);
-          ki = mma(aih, bl[c], ki);
-          kr = mma(arl, bh[c], kr);
-          ki = mma(ail, bh[c], ki);
+          kr = mma(arh[r], bh, kr);
+          ki = mma(aih[r], bh, ki);
+          kr = mma(arh[r], bl, kr);
+          ki = mma(aih[r], bl, ki);
+          kr = mma(arl[r], bh, kr);
+          ki = mma(ail[r], bh, ki);
           acc[kk][j][0] = kr;
           acc[kk][j][1] = ki;
         }
@@ -396,26 +398,25 @@ struct WaveTiles {
 // iteration (one multiplies while the other generates) so that matrix-core and
 // vector work overlap; one barrier per tile separates the buffers' roles.
 template <int NSUB, int ET, int MODE, int W>
-__device__ __forceinline__ void msm_wave(const Problem& p, int sb, int se, int fb, int k0, bool mfma_first,
-                                         char* lds, cf* kpart) {
+__device__ __forceinline__ void msm_wave(const Problem& p, const float4* __restrict__ scat, int sb, int se, int fb,
+                                         int k0, bool mfma_first, char* lds, cf* kpart) {
   using SH = MsmShape<ET, (NSUB > 2 || MODE == MSM_FULLDIR)>;
   constexpr int FBK = SH::FBK;
   using L = FragLds<ET, FBK>;
-  u32x4* const frag = (u32x4*)lds;                    // buffer b at frag + b * per_buf
-  uint2* const masks = (uint2*)(lds + L::frag_bytes);  // buffer b at masks + 4 b
+  u32x4* const frag = (u32x4*)lds;  // buffer b at frag + b * per_buf
   const ElemData<NSUB> ed = load_element<NSUB, ET>(p);
   const BlockScalars bs = block_scalars(p, p.f0 + (double)(fb * FBK) * p.df);
   WaveTiles<ET, FBK, SH::KPW, W> t;
   t.zero();
   int buf = 0;
-  if (sb < se) gen_fragments<NSUB, FBK, MODE, ET>(p, ed, bs, sb, se, frag, masks);
+  if (sb < se) gen_fragments<NSUB, FBK, MODE, ET>(p, ed, bs, scat, sb, se, frag);
   __syncthreads();
   for (int s0 = sb; s0 < se; s0 += MST) {
     const bool more = s0 + MST < se;
     const int nb = buf ^ 1;
-    if (mfma_first) t.step(frag + buf * L::per_buf, masks + 4 * buf, k0);
-    if (more) gen_fragments<NSUB, FBK, MODE, ET>(p, ed, bs, s0 + MST, se, frag + nb * L::per_buf, masks + 4 * nb);
-    if (!mfma_first) t.step(frag + buf * L::per_buf, masks + 4 * buf, k0);
+    if (mfma_first) t.step(frag + buf * L::per_buf, k0);
+    if (more) gen_fragments<NSUB, FBK, MODE, ET>(p, ed, bs, scat, s0 + MST, se, frag + nb * L::per_buf);
+    if (!mfma_first) t.step(frag + buf * L::per_buf, k0);
     __syncthreads();
     buf = nb;
   }
@@ -427,8 +428,8 @@ __device__ __forceinline__ void msm_wave(const Problem& p, int sb, int se, int f
 // w % WPP. Waves w and w + 4 (and w, w + 1) take opposite orders of multiply and
 // generate.
 template <int NSUB, int ET, int MODE>
-__global__ __launch_bounds__(64 * ET, ET == 8 ? 1 : 2) void k_msm(Problem p, int s_per_chunk, int nfpad,
-                                                                 cf* __restrict__ kpart) {
+__global__ __launch_bounds__(64 * ET, ET == 8 ? 1 : 2) void k_msm(Problem p, const float4* __restrict__ scat,
+                                                                 int s_per_chunk, int nfpad, cf* __restrict__ kpart) {
   using SH = MsmShape<ET, (NSUB > 2 || MODE == MSM_FULLDIR)>;
   extern __shared__ __attribute__((aligned(16))) char lds_raw[];
   const int fb = blockIdx.x;
@@ -441,16 +442,16 @@ __global__ __launch_bounds__(64 * ET, ET == 8 ? 1 : 2) void k_msm(Problem p, int
   const bool mfma_first = ((w ^ (w >> 2)) & 1) == 0;
   if constexpr (SH::WPP == 4) {
     switch (w % 4) {
-      case 0: msm_wave<NSUB, ET, MODE, 0>(p, sb, se, fb, k0, mfma_first, lds_raw, kp); break;
-      case 1: msm_wave<NSUB, ET, MODE, 1>(p, sb, se, fb, k0, mfma_first, lds_raw, kp); break;
-      case 2: msm_wave<NSUB, ET, MODE, 2>(p, sb, se, fb, k0, mfma_first, lds_raw, kp); break;
-      default: msm_wave<NSUB, ET, MODE, 3>(p, sb, se, fb, k0, mfma_first, lds_raw, kp); break;
+      case 0: msm_wave<NSUB, ET, MODE, 0>(p, scat, sb, se, fb, k0, mfma_first, lds_raw, kp); break;
+      case 1: msm_wave<NSUB, ET, MODE, 1>(p, scat, sb, se, fb, k0, mfma_first, lds_raw, kp); break;
+      case 2: msm_wave<NSUB, ET, MODE, 2>(p, scat, sb, se, fb, k0, mfma_first, lds_raw, kp); break;
+      default: msm_wave<NSUB, ET, MODE, 3>(p, scat, sb, se, fb, k0, mfma_first, lds_raw, kp); break;
     }
   } else {
     if (w % 2 == 0)
-      msm_wave<NSUB, ET, MODE, 0>(p, sb, se, fb, k0, mfma_first, lds_raw, kp);
+      msm_wave<NSUB, ET, MODE, 0>(p, scat, sb, se, fb, k0, mfma_first, lds_raw, kp);
     else
-      msm_wave<NSUB, ET, MODE, 1>(p, sb, se, fb, k0, mfma_first, lds_raw, kp);
+      msm_wave<NSUB, ET, MODE, 1>(p, scat, sb, se, fb, k0, mfma_first, lds_raw, kp);
   }
 }
 
@@ -531,7 +532,7 @@ __global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, int nch
 // ---------------------------------------------------------------- host side
 struct MsmPlan {
   int ET, FBK, nfb, nfpad, chunks, s_per_chunk, threads;
-  size_t lds, kpart_bytes;
+  size_t lds, kpart_bytes, scat_offset, ws_bytes;
 };
 
 static int device_cus() {
@@ -574,6 +575,8 @@ static MsmPlan msm_plan_et(const Problem& p) {
   m.chunks = std::max(1, (p.S + m.s_per_chunk - 1) / m.s_per_chunk);
   const size_t ep = (size_t)ET * 16;
   m.kpart_bytes = (size_t)m.chunks * m.nfpad * ep * ep * sizeof(cf);
+  m.scat_offset = (m.kpart_bytes + 255) / 256 * 256;
+  m.ws_bytes = m.scat_offset + ((size_t)p.S * sizeof(float4) + 255) / 256 * 256;
   m.lds = FragLds<ET, SH::FBK>::bytes;
   return m;
 }
@@ -586,43 +589,46 @@ static MsmPlan msm_plan(const Problem& p) {
 
 size_t mfma_workspace(const Problem& p) {
   if (p.E > 128 || p.NS > 4) return 256;
-  return (msm_plan(p).kpart_bytes + 255) / 256 * 256;
+  return std::max<size_t>(256, msm_plan(p).ws_bytes);
 }
 
 template <int NSUB, int ET, int MODE>
-static int launch_msm(const Problem& p, const MsmPlan& m, cf* kpart, hipStream_t st) {
+static int launch_msm(const Problem& p, const MsmPlan& m, const float4* scat, cf* kpart, hipStream_t st) {
   auto kern = k_msm<NSUB, ET, MODE>;
   if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds) != hipSuccess)
     return FS_ERR_RESOURCE;
   const int slot = timing_start(st);
-  hipLaunchKernelGGL(kern, dim3(m.nfb, m.chunks), dim3(m.threads), m.lds, st, p, m.s_per_chunk, m.nfpad, kpart);
+  hipLaunchKernelGGL(kern, dim3(m.nfb, m.chunks), dim3(m.threads), m.lds, st, p, scat, m.s_per_chunk, m.nfpad,
+                     kpart);
   timing_stop(slot, st);
   return FS_OK;
 }
 
 template <int NSUB, int ET>
-static int launch_msm_fd(const Problem& p, const MsmPlan& m, cf* kpart, hipStream_t st) {
-  if (p.fulldir) return launch_msm<NSUB, ET, MSM_FULLDIR>(p, m, kpart, st);
-  if (p.baffle == FS_BAFFLE_SOFT && p.kappa <= kPi) return launch_msm<NSUB, ET, MSM_FAST>(p, m, kpart, st);
-  return launch_msm<NSUB, ET, MSM_GENERIC>(p, m, kpart, st);
+static int launch_msm_fd(const Problem& p, const MsmPlan& m, const float4* scat, cf* kpart, hipStream_t st) {
+  if (p.fulldir) return launch_msm<NSUB, ET, MSM_FULLDIR>(p, m, scat, kpart, st);
+  if (p.baffle == FS_BAFFLE_SOFT && p.kappa <= kPi) return launch_msm<NSUB, ET, MSM_FAST>(p, m, scat, kpart, st);
+  return launch_msm<NSUB, ET, MSM_GENERIC>(p, m, scat, kpart, st);
 }
 
 template <int NSUB>
-static int launch_msm_et(const Problem& p, const MsmPlan& m, cf* kpart, hipStream_t st) {
-  return m.ET == 4 ? launch_msm_fd<NSUB, 4>(p, m, kpart, st) : launch_msm_fd<NSUB, 8>(p, m, kpart, st);
+static int launch_msm_et(const Problem& p, const MsmPlan& m, const float4* scat, cf* kpart, hipStream_t st) {
+  return m.ET == 4 ? launch_msm_fd<NSUB, 4>(p, m, scat, kpart, st) : launch_msm_fd<NSUB, 8>(p, m, scat, kpart, st);
 }
 
 int mfma_spectrum(const Problem& p, cf* spect, char* ws, size_t ws_bytes, hipStream_t st) {
   if (p.E > 128) return FS_ERR_UNSUPPORTED;
   const MsmPlan m = msm_plan(p);
-  if (ws_bytes < m.kpart_bytes) return FS_ERR_WORKSPACE;
+  if (ws_bytes < m.ws_bytes) return FS_ERR_WORKSPACE;
   cf* kpart = (cf*)ws;
+  float4* scat = (float4*)(ws + m.scat_offset);
+  if (p.S > 0) hipLaunchKernelGGL(k_scat_prep, dim3((p.S + 255) / 256), dim3(256), 0, st, p, scat);
   int rc;
   switch (p.NS) {
-    case 1: rc = launch_msm_et<1>(p, m, kpart, st); break;
-    case 2: rc = launch_msm_et<2>(p, m, kpart, st); break;
-    case 3: rc = launch_msm_et<3>(p, m, kpart, st); break;
-    case 4: rc = launch_msm_et<4>(p, m, kpart, st); break;
+    case 1: rc = launch_msm_et<1>(p, m, scat, kpart, st); break;
+    case 2: rc = launch_msm_et<2>(p, m, scat, kpart, st); break;
+    case 3: rc = launch_msm_et<3>(p, m, scat, kpart, st); break;
+    case 4: rc = launch_msm_et<4>(p, m, scat, kpart, st); break;
     default: return FS_ERR_UNSUPPORTED;
   }
   if (rc != FS_OK) return rc;
