@@ -71,6 +71,21 @@ def msm_flops_per_launch(n_scat: int, n_elem: int, n_freq: int) -> float:
     return tiles * nfpad * pairs * 6 * (16 * 16 * 32 * 2)
 
 
+TRAFFIC_PROFILE = ROOT / "profiles" / "k_msm_planewave75_200k_traffic.json"
+
+
+def measured_traffic(n_scat: int, n_tx: int, n_freq: int, n_elem: int):
+    """HBM bytes per k_msm launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    (calibrated on a 1 GiB copy), when they were taken on this very workload; else None."""
+    try:
+        d = json.loads(TRAFFIC_PROFILE.read_text())
+    except (OSError, ValueError):
+        return None
+    if d.get("workload") != {"n_scat": n_scat, "n_tx": n_tx, "n_freq": n_freq, "n_elem": n_elem}:
+        return None
+    return d["fetch_bytes"] + d["write_bytes"]
+
+
 def cpu_baseline(params, medium, x, z, rc, delays, plan_o, n_sample: int, seed: int):
     """FP64 NumPy oracle (the reference PYTHON strategy restated) on a bounded sample, 1 thread."""
     from oracle import simus_oracle as O
@@ -102,6 +117,9 @@ def main() -> int:
     ap.add_argument("--plain-tx", type=int, default=3, help="transmits timed with the plain FP32 kernel")
     ap.add_argument("--cpu-sample", type=int, default=1000, help="scatterers in the CPU-baseline / FP64 check sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (calibrates the FETCH_SIZE / WRITE_SIZE "
+                         "counters of a profiler against a known byte count) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -115,6 +133,15 @@ def main() -> int:
     from fastsimus_amd import kernels as K
 
     dev = torch.device("cuda", 0)
+    if args.copy_calib_mib > 0:
+        n = args.copy_calib_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes": n * 4, "launches": 2}), flush=True)
+        return 0
     params, medium, x, z, rc, delays = workload(n_scat=args.n_scat, n_tx=args.n_tx)
     sc = np.stack([x, z], -1)
     plan = fs.simus_precompute(sc, rc, delays, params, medium)
@@ -152,7 +179,7 @@ def main() -> int:
     k_avg = k_ms / max(k_n, 1) / 1e3
     flops = msm_flops_per_launch(S, E, nf)
     roof = {"bound": "mfma", "achieved": flops / k_avg / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
-            "frac": flops / k_avg / PEAK_BF16, "traffic": None,
+            "frac": flops / k_avg / PEAK_BF16, "traffic": measured_traffic(S, T, nf, E),
             "kernel": "k_msm (split-BF16 v_mfma_f32_16x16x32_bf16)", "kernel_ms": k_avg * 1e3, "launches": k_n}
 
     # plain FP32 kernel on a transmit subset of the same inputs
