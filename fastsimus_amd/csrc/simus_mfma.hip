@@ -143,9 +143,24 @@ struct MsmShape {
 };
 
 // Green's-function variants of k_msm: FAST = soft baffle, centre-frequency
-// directivity with |kappa sin(theta)| <= pi (branch-free); GENERIC = any
-// baffle, centre-frequency directivity; FULLDIR = directivity at every frequency.
+// directivity with |kappa sin(theta)| <= 2.5 (branch-free; distances below
+// 1e-30 m are taken as 1e-30 m, which only moves the angle of a scatterer
+// sitting on a sub-element centroid, whose soft-baffle weight is ~1e-16 either
+// way); GENERIC = any baffle, centre-frequency directivity; FULLDIR =
+// directivity at every frequency.
 enum MsmMode { MSM_FAST = 0, MSM_GENERIC = 1, MSM_FULLDIR = 2 };
+
+// sin(a)/a for |a| <= 2.5: Taylor series to a^14 (truncation < 5e-9 on that range).
+__device__ __forceinline__ float sinc_poly_2p5(float a) {
+  const float x = a * a;
+  float r = fmaf(x, 7.6471637e-13f, -1.6059044e-10f);
+  r = fmaf(x, r, 2.5052108e-8f);
+  r = fmaf(x, r, -2.7557319e-6f);
+  r = fmaf(x, r, 1.9841270e-4f);
+  r = fmaf(x, r, -8.3333333e-3f);
+  r = fmaf(x, r, 0.16666667f);
+  return fmaf(-x, r, 1.0f);
+}
 
 // sin(a)/a for |a| <= pi: Taylor series to a^16 (truncation < 1e-8 on that range).
 __device__ __forceinline__ float sinc_poly_pi(float a) {
@@ -237,8 +252,6 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<N
   float re[FBK][4], im[FBK][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-#pragma unroll
-    for (int k = 0; k < FBK; ++k) re[k][i] = im[k][i] = 0.f;
     const int s = s0 + 4 * g + i;
     const bool valid = s < s_end;
     const float4 sd = scat[valid ? s : s_end - 1];  // in-bounds load, masked below
@@ -254,9 +267,15 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<N
       const double szj = held ? ed.sz[j] : p.sub_z[ed.ee * NSUB + j];
       const double dx = xd - sxj;
       const double dz = zd - szj;
-      const double r2 = fma(dx, dx, dz * dz);
-      const bool zero = !(r2 > 0.0);
-      double y = __builtin_amdgcn_rsq(zero ? 1.0 : r2);
+      double r2 = fma(dx, dx, dz * dz);
+      bool zero = false;
+      if constexpr (MODE == MSM_FAST) {
+        r2 = fmax(r2, 1e-60);
+      } else {
+        zero = !(r2 > 0.0);
+        r2 = zero ? 1.0 : r2;
+      }
+      double y = __builtin_amdgcn_rsq(r2);
       y = y * fma(-0.5 * r2, y * y, 1.5);  // Newton step on 1/sqrt
       const double rcl = fmax(zero ? 0.0 : r2 * y, p.min_dist);
       const float rinv = (float)y;
@@ -276,7 +295,7 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<N
       const float rcf = (float)rcl;
       float amp = obl * __builtin_amdgcn_rsqf(rcf) * w;
       if constexpr (MODE == MSM_FAST) {
-        amp *= sinc_poly_pi(p.kappa * st);
+        amp *= sinc_poly_2p5(p.kappa * st);
       } else if constexpr (MODE == MSM_GENERIC) {
         amp *= bs.poly_ok ? sinc_poly_pi(p.kappa * st) : sinc_fast(p.kappa * st);
       }
@@ -291,11 +310,11 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<N
       for (int k = 0; k < FBK; ++k) {
         if constexpr (MODE == MSM_FULLDIR) {
           const float d = sinc_fast(p.kappa_per_hz * fmaf((float)k, bs.df, bs.fb) * st);
-          re[k][i] = fmaf(d, gv.x, re[k][i]);
-          im[k][i] = fmaf(d, gv.y, im[k][i]);
+          re[k][i] = j == 0 ? d * gv.x : fmaf(d, gv.x, re[k][i]);
+          im[k][i] = j == 0 ? d * gv.y : fmaf(d, gv.y, im[k][i]);
         } else {
-          re[k][i] += gv.x;
-          im[k][i] += gv.y;
+          re[k][i] = j == 0 ? gv.x : re[k][i] + gv.x;
+          im[k][i] = j == 0 ? gv.y : im[k][i] + gv.y;
         }
         if (k + 1 < FBK) gv = cmul(gv, q);
       }
@@ -607,7 +626,7 @@ static int launch_msm(const Problem& p, const MsmPlan& m, const float4* scat, cf
 template <int NSUB, int ET>
 static int launch_msm_fd(const Problem& p, const MsmPlan& m, const float4* scat, cf* kpart, hipStream_t st) {
   if (p.fulldir) return launch_msm<NSUB, ET, MSM_FULLDIR>(p, m, scat, kpart, st);
-  if (p.baffle == FS_BAFFLE_SOFT && p.kappa <= kPi) return launch_msm<NSUB, ET, MSM_FAST>(p, m, scat, kpart, st);
+  if (p.baffle == FS_BAFFLE_SOFT && p.kappa <= 2.5f) return launch_msm<NSUB, ET, MSM_FAST>(p, m, scat, kpart, st);
   return launch_msm<NSUB, ET, MSM_GENERIC>(p, m, scat, kpart, st);
 }
 
