@@ -23,6 +23,8 @@ EXPORTS = (
     "fs_status_string",
     "fs_simus_workspace_bytes",
     "fs_simus_spectrum",
+    "fs_simus_batch_workspace_bytes",
+    "fs_simus_spectrum_batch",
     "fs_rf_workspace_bytes",
     "fs_spectrum_to_rf",
     "fs_rf_adjoint_workspace_bytes",
@@ -73,6 +75,11 @@ def load() -> C.CDLL:
     lib.fs_simus_workspace_bytes.argtypes = [D, C.c_int]
     lib.fs_simus_spectrum.restype = C.c_int
     lib.fs_simus_spectrum.argtypes = [D, C.c_int, P, P, C.c_size_t, P]
+    I64 = C.POINTER(C.c_int64)
+    lib.fs_simus_batch_workspace_bytes.restype = C.c_size_t
+    lib.fs_simus_batch_workspace_bytes.argtypes = [D, C.c_int, C.c_int, I64]
+    lib.fs_simus_spectrum_batch.restype = C.c_int
+    lib.fs_simus_spectrum_batch.argtypes = [D, C.c_int, C.c_int, I64, P, P, C.c_size_t, P]
     lib.fs_rf_workspace_bytes.restype = C.c_size_t
     lib.fs_rf_workspace_bytes.argtypes = [C.c_int]
     lib.fs_spectrum_to_rf.restype = C.c_int

@@ -12,6 +12,9 @@ int fp32_spectrum(const Problem& p, cf* spect, char* ws, hipStream_t st);
 size_t fp32_workspace(const Problem& p);
 int mfma_spectrum(const Problem& p, cf* spect, char* ws, size_t ws_bytes, hipStream_t st);
 size_t mfma_workspace(const Problem& p);
+int mfma_spectrum_batch(const Problem& p, int P, const int64_t* off, cf* spect, char* ws, size_t ws_bytes,
+                        hipStream_t st);
+size_t mfma_workspace_batch(const Problem& p, int P, const int64_t* off);
 int spectrum_to_rf(const cf* spect, int NT, int NF, int E, int nff, int idx0, int N, cf* full, float* rf, float* rf_u,
                    unsigned* peak, hipStream_t st);
 size_t rf_adjoint_ws(int NT, int nkeep, int E);
@@ -114,6 +117,71 @@ int fs_simus_spectrum(const fs_simus_desc* d, int method, float* spect, void* wo
   if (method == FS_METHOD_MFMA_BF16X3) {
     if (!workspace || workspace_bytes < mfma_workspace(p)) return FS_ERR_WORKSPACE;
     return mfma_spectrum(p, (cf*)spect, (char*)workspace, workspace_bytes, st);
+  }
+  return FS_ERR_INVALID;
+}
+
+static bool valid_offsets(const Problem& p, int P, const int64_t* off) {
+  if (P < 1 || P > 192 || !off || off[0] != 0 || off[P] != p.S) return false;
+  for (int q = 0; q < P; ++q)
+    if (off[q + 1] < off[q]) return false;
+  return true;
+}
+
+// Phantom q of a batch as its own problem (scatterer pointers offset into the concatenation).
+static Problem phantom_problem(const Problem& p, const int64_t* off, int q) {
+  Problem s = p;
+  s.S = (int)(off[q + 1] - off[q]);
+  s.x = p.x + off[q];
+  s.z = p.z + off[q];
+  s.rc = p.rc + off[q];
+  return s;
+}
+
+size_t fs_simus_batch_workspace_bytes(const fs_simus_desc* d, int method, int n_phantoms,
+                                      const int64_t* phantom_offsets) {
+  Problem p;
+  if (make_problem(d, &p, true, true) != FS_OK || !valid_offsets(p, n_phantoms, phantom_offsets)) return 0;
+  if (method == FS_METHOD_FP32) {
+    size_t m = 256;
+    for (int q = 0; q < n_phantoms; ++q) {
+      const Problem s = phantom_problem(p, phantom_offsets, q);
+      if (s.S > 0) m = std::max(m, fp32_workspace(s));
+    }
+    return m;
+  }
+  if (method == FS_METHOD_MFMA_BF16X3) return mfma_workspace_batch(p, n_phantoms, phantom_offsets);
+  return 0;
+}
+
+int fs_simus_spectrum_batch(const fs_simus_desc* d, int method, int n_phantoms, const int64_t* phantom_offsets,
+                            float* spect, void* workspace, size_t workspace_bytes, void* stream) {
+  Problem p;
+  int rc = make_problem(d, &p, true, true);
+  if (rc != FS_OK) return rc;
+  if (!spect) return FS_ERR_INVALID;
+  if (n_phantoms > 192) return FS_ERR_UNSUPPORTED;
+  if (!valid_offsets(p, n_phantoms, phantom_offsets)) return FS_ERR_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  if (method == FS_METHOD_FP32) {
+    const size_t per = 2 * (size_t)p.NT * p.NF * p.E;
+    for (int q = 0; q < n_phantoms; ++q) {
+      const Problem s = phantom_problem(p, phantom_offsets, q);
+      float* out = spect + q * per;
+      if (s.S == 0) {
+        rc = fill_zero(out, per, st);
+      } else {
+        if (!workspace || workspace_bytes < fp32_workspace(s)) return FS_ERR_WORKSPACE;
+        rc = fp32_spectrum(s, (cf*)out, (char*)workspace, st);
+      }
+      if (rc != FS_OK) return rc;
+    }
+    return FS_OK;
+  }
+  if (method == FS_METHOD_MFMA_BF16X3) {
+    if (!workspace || workspace_bytes < mfma_workspace_batch(p, n_phantoms, phantom_offsets))
+      return FS_ERR_WORKSPACE;
+    return mfma_spectrum_batch(p, n_phantoms, phantom_offsets, (cf*)spect, (char*)workspace, workspace_bytes, st);
   }
   return FS_ERR_INVALID;
 }

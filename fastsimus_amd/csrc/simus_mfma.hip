@@ -38,6 +38,8 @@
 // k_msm_apply sums the chunks in a fixed order and applies every transmit
 // (E x E x n_tx per frequency, FP32).
 #include <algorithm>
+#include <cmath>
+#include <vector>
 
 #include "fs_device.h"
 
@@ -262,7 +264,7 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<N
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
       // sub-element centroids: held in registers where they fit, else re-read (cached)
-      constexpr bool held = NSUB <= 2 && MODE != MSM_FULLDIR;
+      constexpr bool held = NSUB <= 2 && MODE == MSM_FAST;
       const double sxj = held ? ed.sx[j] : p.sub_x[ed.ee * NSUB + j];
       const double szj = held ? ed.sz[j] : p.sub_z[ed.ee * NSUB + j];
       const double dx = xd - sxj;
@@ -442,47 +444,94 @@ __device__ __forceinline__ void msm_wave(const Problem& p, const float4* __restr
   t.store(kpart, ET * 16, fb * FBK + k0);
 }
 
-// Grid (frequency block, scatterer chunk); ET waves per workgroup. Wave w owns
-// frequencies KPW (w / WPP) .. KPW (w / WPP) + KPW - 1 of the block and job list
-// w % WPP. Waves w and w + 4 (and w, w + 1) take opposite orders of multiply and
-// generate.
+// ---------------------------------------------------------------- work units
+// A unit is a contiguous scatterer range of one phantom with its own partial
+// K slot; an item is (unit, frequency block). Units are built on the host,
+// largest first, and passed by value (so that a captured graph replays them).
+struct MsmUnit {
+  int s_begin, s_end, slot, pad;
+};
+constexpr int MSM_MAX_UNITS = 192;
+struct MsmUnits {
+  int n;    // units
+  int nfb;  // frequency blocks
+  int pad0, pad1;
+  MsmUnit u[MSM_MAX_UNITS];
+};
+constexpr int MSM_MAX_PHANTOMS = 192;
+struct MsmPhantoms {
+  int n, pad;
+  int2 ph[MSM_MAX_PHANTOMS];  // (first K slot, slot count) of each phantom
+};
+
+// Persistent: every workgroup takes items from a queue (atomic counter, reset
+// by the host before the launch) until none is left; items come largest unit
+// first. Every wave runs the same sequence of barriers: two per item, one
+// before its tile loop and one per tile.
+template <int NSUB, int ET, int MODE, int W>
+__device__ __forceinline__ void msm_persistent(const Problem& p, const float4* __restrict__ scat,
+                                               const MsmUnits& units, int* __restrict__ queue, int nfpad,
+                                               cf* __restrict__ kpart, int k0, bool mfma_first, char* lds,
+                                               int* next_item) {
+  const int n_items = units.n * units.nfb;
+  for (;;) {
+    if (threadIdx.x == 0) *next_item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int item = __builtin_amdgcn_readfirstlane(*next_item);
+    __syncthreads();
+    if (item >= n_items) break;  // uniform over the workgroup
+    const int ui = item / units.nfb;
+    const int fb = item - ui * units.nfb;
+    const MsmUnit u = units.u[ui];
+    cf* kp = kpart + (size_t)u.slot * nfpad * (ET * 16) * (ET * 16);
+    msm_wave<NSUB, ET, MODE, W>(p, scat, u.s_begin, u.s_end, fb, k0, mfma_first, lds, kp);
+  }
+}
+
+// ET waves per workgroup; wave w owns frequencies KPW (w / WPP) ..
+// KPW (w / WPP) + KPW - 1 of each item's block and job list w % WPP. Waves w
+// and w + 4 (and w, w + 1) take opposite orders of multiply and generate.
 template <int NSUB, int ET, int MODE>
 __global__ __launch_bounds__(64 * ET, ET == 8 ? 1 : 2) void k_msm(Problem p, const float4* __restrict__ scat,
-                                                                 int s_per_chunk, int nfpad, cf* __restrict__ kpart) {
+                                                                 const MsmUnits units, int* __restrict__ queue,
+                                                                 int nfpad, cf* __restrict__ kpart) {
   using SH = MsmShape<ET, (NSUB > 2 || MODE == MSM_FULLDIR)>;
   extern __shared__ __attribute__((aligned(16))) char lds_raw[];
-  const int fb = blockIdx.x;
-  const int chunk = blockIdx.y;
-  const int sb = chunk * s_per_chunk;
-  const int se = min(sb + s_per_chunk, p.S);
-  cf* kp = kpart + (size_t)chunk * nfpad * (ET * 16) * (ET * 16);
+  __shared__ int next_item;
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int k0 = SH::KPW * (w / SH::WPP);
   const bool mfma_first = ((w ^ (w >> 2)) & 1) == 0;
   if constexpr (SH::WPP == 4) {
     switch (w % 4) {
-      case 0: msm_wave<NSUB, ET, MODE, 0>(p, scat, sb, se, fb, k0, mfma_first, lds_raw, kp); break;
-      case 1: msm_wave<NSUB, ET, MODE, 1>(p, scat, sb, se, fb, k0, mfma_first, lds_raw, kp); break;
-      case 2: msm_wave<NSUB, ET, MODE, 2>(p, scat, sb, se, fb, k0, mfma_first, lds_raw, kp); break;
-      default: msm_wave<NSUB, ET, MODE, 3>(p, scat, sb, se, fb, k0, mfma_first, lds_raw, kp); break;
+      case 0: msm_persistent<NSUB, ET, MODE, 0>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
+      case 1: msm_persistent<NSUB, ET, MODE, 1>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
+      case 2: msm_persistent<NSUB, ET, MODE, 2>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
+      default: msm_persistent<NSUB, ET, MODE, 3>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
     }
   } else {
     if (w % 2 == 0)
-      msm_wave<NSUB, ET, MODE, 0>(p, scat, sb, se, fb, k0, mfma_first, lds_raw, kp);
+      msm_persistent<NSUB, ET, MODE, 0>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item);
     else
-      msm_wave<NSUB, ET, MODE, 1>(p, scat, sb, se, fb, k0, mfma_first, lds_raw, kp);
+      msm_persistent<NSUB, ET, MODE, 1>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item);
   }
+}
+
+__global__ void k_set_int(int* __restrict__ ptr, int value) {
+  if (threadIdx.x == 0) *ptr = value;
 }
 
 // ---------------------------------------------------------------- apply all transmits
 // out[t][k][e] = corr probe_k PP_k sum_e' K_k[e][e'] A_t(k, e'),  K_k = sum_c Kpart[c][k] (upper tiles, symmetric)
 constexpr int AP_CH = 32;  // e' per LDS chunk
 
-__global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, int nchunks, int nfpad,
-                                                   const cf* __restrict__ kpart, cf* __restrict__ out) {
+// Grid (frequency, phantom); phantom ph sums its K slots and writes out[ph][t][k][e].
+__global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, const MsmPhantoms phantoms, int nfpad,
+                                                   const cf* __restrict__ kpart, cf* __restrict__ out_all) {
   __shared__ cf Ks[128][AP_CH + 1];
   __shared__ cf As[AP_CH][2 * 40 + 1];
   const int k = blockIdx.x;
+  const int slot0 = phantoms.ph[blockIdx.y].x, nchunks = phantoms.ph[blockIdx.y].y;
+  cf* __restrict__ out = out_all + (size_t)blockIdx.y * p.NT * p.NF * p.E;
   const int tid = threadIdx.x;
   const int e = tid & 127;
   const int tg = tid >> 7;  // transmit parity
@@ -502,7 +551,7 @@ __global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, int nch
           const int ra = ((r >> 4) <= (col >> 4)) ? r : col;
           const int cb = ((r >> 4) <= (col >> 4)) ? col : r;
           float sx = 0.f, sy = 0.f;
-          for (int c = 0; c < nchunks; ++c) {
+          for (int c = slot0; c < slot0 + nchunks; ++c) {
             const cf w = kpart[(((size_t)c * nfpad + k) * E_pad + ra) * E_pad + cb];
             sx += w.x;
             sy += w.y;
@@ -550,20 +599,35 @@ __global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, int nch
 
 // ---------------------------------------------------------------- host side
 struct MsmPlan {
-  int ET, FBK, nfb, nfpad, chunks, s_per_chunk, threads;
-  size_t lds, kpart_bytes, scat_offset, ws_bytes;
+  int ET, FBK, nfb, nfpad, threads, grid, n_items;
+  size_t lds, kpart_bytes, scat_offset, queue_offset, ws_bytes;
+  MsmUnits units;
+  MsmPhantoms phantoms;
 };
 
-static int device_cus() {
-  int dev = 0, cus = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-    return 256;
-  return cus;
+constexpr int MAX_DEVICES = 64;
+
+static int current_device() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return 0;
+  return dev;
 }
 
-// Scatterer chunks per frequency block: the count whose last round of
-// workgroups is fullest (rounds / chunks smallest), at least 8 tiles per chunk.
+// CU count of the current device, queried once per device (not a stream operation, so safe
+// while a stream is being captured).
+static int device_cus() {
+  static int cached[MAX_DEVICES] = {0};
+  const int dev = current_device();
+  if (cached[dev] <= 0) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    cached[dev] = cus;
+  }
+  return cached[dev];
+}
+
+// Scatterer chunks of a single phantom: the count whose last round of
+// equal items is fullest (rounds / chunks smallest), at least 8 tiles each.
 static int pick_chunks(int nfb, int tiles, int slots) {
   const int cmax = std::max(1, std::min(16, tiles / 8));
   double best = 1e30;
@@ -579,80 +643,152 @@ static int pick_chunks(int nfb, int tiles, int slots) {
   return best_c;
 }
 
+// Units of a batch of phantoms [off[q], off[q + 1]): one phantom is split as
+// pick_chunks says; in a batch every unit stays below 1/8 of a queue slot's
+// share of the work, so that the largest-first queue ends evenly.
 template <int ET, bool HEAVY>
-static MsmPlan msm_plan_et(const Problem& p) {
+static int msm_plan_et(const Problem& p, int P, const int64_t* off, MsmPlan* out) {
   using SH = MsmShape<ET, HEAVY>;
-  MsmPlan m{};
+  MsmPlan& m = *out;
+  m = MsmPlan{};
   m.ET = ET;
   m.FBK = SH::FBK;
   m.threads = SH::THREADS;
   m.nfb = (p.NF + m.FBK - 1) / m.FBK;
   m.nfpad = m.nfb * m.FBK;
-  const int tiles = std::max(1, (p.S + MST - 1) / MST);
-  const int chunks = pick_chunks(m.nfb, tiles, device_cus() * SH::WG_PER_CU);
-  m.s_per_chunk = ((tiles + chunks - 1) / chunks) * MST;
-  m.chunks = std::max(1, (p.S + m.s_per_chunk - 1) / m.s_per_chunk);
+  const int slots = device_cus() * SH::WG_PER_CU;
+  std::vector<int> tiles(P), chunks(P);
+  long long total = 0;
+  for (int q = 0; q < P; ++q) {
+    tiles[q] = (int)((off[q + 1] - off[q] + MST - 1) / MST);
+    total += std::max(tiles[q], 1);
+  }
+  if (P == 1) {
+    chunks[0] = pick_chunks(m.nfb, std::max(tiles[0], 1), slots);
+  } else {
+    double tmax = std::max(8.0, (double)total * m.nfb / (slots * 8.0));
+    for (;;) {
+      int n = 0;
+      for (int q = 0; q < P; ++q) {
+        chunks[q] = std::max(1, (int)std::ceil(tiles[q] / tmax));
+        n += chunks[q];
+      }
+      if (n <= MSM_MAX_UNITS) break;
+      tmax *= 1.25;
+    }
+  }
+  std::vector<MsmUnit> us;
+  m.phantoms.n = P;
+  for (int q = 0; q < P; ++q) {
+    const int per = std::max(1, (tiles[q] + chunks[q] - 1) / chunks[q]) * MST;
+    m.phantoms.ph[q] = make_int2((int)us.size(), chunks[q]);
+    for (int c = 0; c < chunks[q]; ++c) {
+      const long long b = std::min<long long>(off[q] + (long long)c * per, off[q + 1]);
+      const long long e = std::min<long long>(b + per, off[q + 1]);
+      us.push_back(MsmUnit{(int)b, (int)e, (int)us.size(), 0});
+    }
+  }
+  std::stable_sort(us.begin(), us.end(),
+                   [](const MsmUnit& a, const MsmUnit& b) { return a.s_end - a.s_begin > b.s_end - b.s_begin; });
+  m.units.n = (int)us.size();
+  m.units.nfb = m.nfb;
+  for (size_t i = 0; i < us.size(); ++i) m.units.u[i] = us[i];
+  m.n_items = m.units.n * m.nfb;
+  m.grid = std::max(1, std::min(m.n_items, slots));
   const size_t ep = (size_t)ET * 16;
-  m.kpart_bytes = (size_t)m.chunks * m.nfpad * ep * ep * sizeof(cf);
+  m.kpart_bytes = (size_t)m.units.n * m.nfpad * ep * ep * sizeof(cf);
   m.scat_offset = (m.kpart_bytes + 255) / 256 * 256;
-  m.ws_bytes = m.scat_offset + ((size_t)p.S * sizeof(float4) + 255) / 256 * 256;
+  m.queue_offset = m.scat_offset + ((size_t)off[P] * sizeof(float4) + 255) / 256 * 256;
+  m.ws_bytes = m.queue_offset + 256;
   m.lds = FragLds<ET, SH::FBK>::bytes;
-  return m;
+  return FS_OK;
 }
 
-static MsmPlan msm_plan(const Problem& p) {
+static int msm_plan(const Problem& p, int P, const int64_t* off, MsmPlan* m) {
+  if (P < 1 || P > MSM_MAX_PHANTOMS) return FS_ERR_UNSUPPORTED;
+  if (off[0] != 0 || off[P] != p.S) return FS_ERR_INVALID;
+  for (int q = 0; q < P; ++q)
+    if (off[q + 1] < off[q]) return FS_ERR_INVALID;
   const bool heavy = p.NS > 2 || p.fulldir;
-  if (p.E <= 64) return heavy ? msm_plan_et<4, true>(p) : msm_plan_et<4, false>(p);
-  return heavy ? msm_plan_et<8, true>(p) : msm_plan_et<8, false>(p);
+  if (p.E <= 64) return heavy ? msm_plan_et<4, true>(p, P, off, m) : msm_plan_et<4, false>(p, P, off, m);
+  return heavy ? msm_plan_et<8, true>(p, P, off, m) : msm_plan_et<8, false>(p, P, off, m);
+}
+
+size_t mfma_workspace_batch(const Problem& p, int P, const int64_t* off) {
+  if (p.E > 128 || p.NS > 4) return 256;
+  MsmPlan m;
+  if (msm_plan(p, P, off, &m) != FS_OK) return 0;
+  return m.ws_bytes;
 }
 
 size_t mfma_workspace(const Problem& p) {
-  if (p.E > 128 || p.NS > 4) return 256;
-  return std::max<size_t>(256, msm_plan(p).ws_bytes);
+  const int64_t off[2] = {0, p.S};
+  return mfma_workspace_batch(p, 1, off);
 }
 
 template <int NSUB, int ET, int MODE>
-static int launch_msm(const Problem& p, const MsmPlan& m, const float4* scat, cf* kpart, hipStream_t st) {
+static int launch_msm(const Problem& p, const MsmPlan& m, const float4* scat, int* queue, cf* kpart,
+                      hipStream_t st) {
   auto kern = k_msm<NSUB, ET, MODE>;
-  if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds) != hipSuccess)
-    return FS_ERR_RESOURCE;
+  static bool lds_set[MAX_DEVICES] = {false};  // dynamic LDS above 64 KiB: set once per device
+  const int dev = current_device();
+  if (!lds_set[dev]) {
+    if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)m.lds) != hipSuccess)
+      return FS_ERR_RESOURCE;
+    lds_set[dev] = true;
+  }
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, queue, 0);
   const int slot = timing_start(st);
-  hipLaunchKernelGGL(kern, dim3(m.nfb, m.chunks), dim3(m.threads), m.lds, st, p, scat, m.s_per_chunk, m.nfpad,
-                     kpart);
+  hipLaunchKernelGGL(kern, dim3(m.grid), dim3(m.threads), m.lds, st, p, scat, m.units, queue, m.nfpad, kpart);
   timing_stop(slot, st);
   return FS_OK;
 }
 
 template <int NSUB, int ET>
-static int launch_msm_fd(const Problem& p, const MsmPlan& m, const float4* scat, cf* kpart, hipStream_t st) {
-  if (p.fulldir) return launch_msm<NSUB, ET, MSM_FULLDIR>(p, m, scat, kpart, st);
-  if (p.baffle == FS_BAFFLE_SOFT && p.kappa <= 2.5f) return launch_msm<NSUB, ET, MSM_FAST>(p, m, scat, kpart, st);
-  return launch_msm<NSUB, ET, MSM_GENERIC>(p, m, scat, kpart, st);
+static int launch_msm_fd(const Problem& p, const MsmPlan& m, const float4* scat, int* queue, cf* kpart,
+                         hipStream_t st) {
+  if (p.fulldir) return launch_msm<NSUB, ET, MSM_FULLDIR>(p, m, scat, queue, kpart, st);
+  if (p.baffle == FS_BAFFLE_SOFT && p.kappa <= 2.5f)
+    return launch_msm<NSUB, ET, MSM_FAST>(p, m, scat, queue, kpart, st);
+  return launch_msm<NSUB, ET, MSM_GENERIC>(p, m, scat, queue, kpart, st);
 }
 
 template <int NSUB>
-static int launch_msm_et(const Problem& p, const MsmPlan& m, const float4* scat, cf* kpart, hipStream_t st) {
-  return m.ET == 4 ? launch_msm_fd<NSUB, 4>(p, m, scat, kpart, st) : launch_msm_fd<NSUB, 8>(p, m, scat, kpart, st);
+static int launch_msm_et(const Problem& p, const MsmPlan& m, const float4* scat, int* queue, cf* kpart,
+                         hipStream_t st) {
+  return m.ET == 4 ? launch_msm_fd<NSUB, 4>(p, m, scat, queue, kpart, st)
+                   : launch_msm_fd<NSUB, 8>(p, m, scat, queue, kpart, st);
 }
 
-int mfma_spectrum(const Problem& p, cf* spect, char* ws, size_t ws_bytes, hipStream_t st) {
+// Spectra of P phantoms sharing one probe, transmit set and frequency plan:
+// spect[ph][t][k][e]. Only kernel launches (no host synchronisation, no memset),
+// so that the call can be captured in a graph.
+int mfma_spectrum_batch(const Problem& p, int P, const int64_t* off, cf* spect, char* ws, size_t ws_bytes,
+                        hipStream_t st) {
   if (p.E > 128) return FS_ERR_UNSUPPORTED;
-  const MsmPlan m = msm_plan(p);
+  MsmPlan m;
+  int rc = msm_plan(p, P, off, &m);
+  if (rc != FS_OK) return rc;
   if (ws_bytes < m.ws_bytes) return FS_ERR_WORKSPACE;
   cf* kpart = (cf*)ws;
   float4* scat = (float4*)(ws + m.scat_offset);
+  int* queue = (int*)(ws + m.queue_offset);
   if (p.S > 0) hipLaunchKernelGGL(k_scat_prep, dim3((p.S + 255) / 256), dim3(256), 0, st, p, scat);
-  int rc;
   switch (p.NS) {
-    case 1: rc = launch_msm_et<1>(p, m, scat, kpart, st); break;
-    case 2: rc = launch_msm_et<2>(p, m, scat, kpart, st); break;
-    case 3: rc = launch_msm_et<3>(p, m, scat, kpart, st); break;
-    case 4: rc = launch_msm_et<4>(p, m, scat, kpart, st); break;
+    case 1: rc = launch_msm_et<1>(p, m, scat, queue, kpart, st); break;
+    case 2: rc = launch_msm_et<2>(p, m, scat, queue, kpart, st); break;
+    case 3: rc = launch_msm_et<3>(p, m, scat, queue, kpart, st); break;
+    case 4: rc = launch_msm_et<4>(p, m, scat, queue, kpart, st); break;
     default: return FS_ERR_UNSUPPORTED;
   }
   if (rc != FS_OK) return rc;
-  hipLaunchKernelGGL(k_msm_apply, dim3(p.NF), dim3(256), 0, st, p, m.ET * 16, m.chunks, m.nfpad, kpart, spect);
+  hipLaunchKernelGGL(k_msm_apply, dim3(p.NF, P), dim3(256), 0, st, p, m.ET * 16, m.phantoms, m.nfpad, kpart, spect);
   return hipGetLastError() == hipSuccess ? FS_OK : FS_ERR_LAUNCH;
+}
+
+int mfma_spectrum(const Problem& p, cf* spect, char* ws, size_t ws_bytes, hipStream_t st) {
+  const int64_t off[2] = {0, p.S};
+  return mfma_spectrum_batch(p, 1, off, spect, ws, ws_bytes, st);
 }
 
 }  // namespace fs

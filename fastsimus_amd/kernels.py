@@ -116,6 +116,14 @@ class Problem:
     def dref(self):
         return C.byref(self.desc)
 
+    def bind_scatterers(self, x: torch.Tensor, z: torch.Tensor, rc: torch.Tensor | None) -> None:
+        """Point the descriptor at caller-owned float32 device tensors (e.g. the static inputs of a graph)."""
+        for t in (x, z) + ((rc,) if rc is not None else ()):
+            if t.dtype != torch.float32 or t.device != self.device or not t.is_contiguous() or t.numel() != self.x.numel():
+                raise ValueError("scatterer tensors must be contiguous float32 on the problem's device, same length")
+        self.x, self.z, self.rc = x, z, rc
+        self.desc.scat_x, self.desc.scat_z, self.desc.rc = _ptr(x), _ptr(z), _ptr(rc)
+
 
 def _stream(device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
@@ -136,6 +144,36 @@ def run_spectrum(prob: Problem, method: str = "fp32") -> torch.Tensor:
     out = torch.empty((prob.n_tx, prob.n_freq, prob.params.n_elements, 2), dtype=torch.float32, device=prob.device)
     st = lib.fs_simus_spectrum(prob.dref, m, out.data_ptr(), ws.data_ptr(), ws.numel(), _stream(prob.device))
     _lib.check(st, "fs_simus_spectrum")
+    return torch.view_as_complex(out)
+
+
+def _offsets_ptr(offsets: np.ndarray):
+    return offsets.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def batch_workspace(prob: Problem, offsets: np.ndarray, method: str = "mfma") -> torch.Tensor:
+    """Workspace of ``fs_simus_spectrum_batch`` for these phantom offsets."""
+    lib = _lib.load()
+    n = lib.fs_simus_batch_workspace_bytes(prob.dref, METHODS[method], len(offsets) - 1, _offsets_ptr(offsets))
+    if n == 0:
+        _lib.check(_lib.FS_ERR_INVALID, "fs_simus_batch_workspace_bytes")
+    return _workspace(n, prob.device)
+
+
+def run_spectrum_batch(prob: Problem, offsets: np.ndarray, method: str = "mfma", *, out: torch.Tensor | None = None,
+                       workspace: torch.Tensor | None = None) -> torch.Tensor:
+    """Spectra of a batch of phantoms (concatenated in ``prob``; phantom q is
+    ``[offsets[q], offsets[q + 1])``): complex64 (n_phantoms, n_tx, n_freq, n_elem), times ``desc.correction``."""
+    lib = _lib.load()
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    P = len(offsets) - 1
+    ws = workspace if workspace is not None else batch_workspace(prob, offsets, method)
+    if out is None:
+        out = torch.empty((P, prob.n_tx, prob.n_freq, prob.params.n_elements, 2), dtype=torch.float32,
+                          device=prob.device)
+    st = lib.fs_simus_spectrum_batch(prob.dref, METHODS[method], P, _offsets_ptr(offsets), out.data_ptr(),
+                                     ws.data_ptr(), ws.numel(), _stream(prob.device))
+    _lib.check(st, "fs_simus_spectrum_batch")
     return torch.view_as_complex(out)
 
 

@@ -101,6 +101,19 @@ size_t fs_simus_workspace_bytes(const fs_simus_desc* d, int method);
 int fs_simus_spectrum(const fs_simus_desc* d, int method, float* spect, void* workspace, size_t workspace_bytes,
                       void* stream);
 
+/* Batch of phantoms sharing the probe, the transmits and the frequency plan of d: the scatterers of
+ * all phantoms are concatenated in d->scat_x / scat_z / rc (d->n_scat in total) and phantom q is
+ * [phantom_offsets[q], phantom_offsets[q + 1]) (host array of n_phantoms + 1 int64, starting at 0
+ * and ending at d->n_scat; 1 <= n_phantoms <= 192). Writes spect[q][t][k][e] (interleaved complex
+ * float32), each phantom exactly as fs_simus_spectrum would on its own scatterers. With
+ * FS_METHOD_MFMA_BF16X3 all phantoms run in one persistent launch (largest work first) and the call
+ * only enqueues kernels, so it can be captured in a graph. No counterpart in the reference, whose
+ * harness calls simus_cuda once per phantom and firing (tests/_picmus_phantom_shared.py:319-367). */
+size_t fs_simus_batch_workspace_bytes(const fs_simus_desc* d, int method, int n_phantoms,
+                                      const int64_t* phantom_offsets);
+int fs_simus_spectrum_batch(const fs_simus_desc* d, int method, int n_phantoms, const int64_t* phantom_offsets,
+                            float* spect, void* workspace, size_t workspace_bytes, void* stream);
+
 /* Place the band into the full spectrum and compute gated RF per transmit:
  *   spectrum_full[t][i][e] (complex64, i < n_ff), zero outside [idx0, idx0 + n_freq);
  *   rf[t][n][e] = gate(irfft(conj(full), n_fft)[n]), n < (n_fft + 1) / 2.
