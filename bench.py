@@ -107,6 +107,177 @@ def cpu_baseline(params, medium, x, z, rc, delays, plan_o, n_sample: int, seed: 
                       f"({os.cpu_count()} logical CPUs visible)"}, idx, t_mid, rf
 
 
+def phantom_workload(seed: int = 2027, n_phantoms: int = 64, n_beams: int = 128):
+    """Uneven phantom batch: 64-element 2.7 MHz phased array (pitch 0.30 mm, width 0.25 mm, 74 % bandwidth),
+    phantoms of 10^U[4, 6] scatterers uniform over a 90-degree sector to 12 cm, rc ~ N(0, 1) float32,
+    beams focused at 6 cm steered over [-40, 40] degrees."""
+    import fastsimus_amd as fs
+
+    params = fs.TransducerParams(freq_center=2.7e6, pitch=3.0e-4, width=2.5e-4, n_elements=64, bandwidth=0.74)
+    medium = fs.MediumParams(speed_of_sound=1540.0)
+    rng = np.random.default_rng(seed)
+    phantoms = []
+    for _ in range(n_phantoms):
+        n = int(round(10 ** rng.uniform(4, 6)))
+        phi = rng.uniform(-np.pi / 4, np.pi / 4, n)
+        rho = 0.12 * np.sqrt(rng.uniform(0, 1, n))
+        pts = np.stack([rho * np.sin(phi), rho * np.cos(phi)], -1).astype(np.float32)
+        phantoms.append((pts, rng.standard_normal(n).astype(np.float32)))
+    pos, _, _ = fs.element_positions(params.n_elements, params.pitch, params.radius)
+    psi = np.deg2rad(np.linspace(-40.0, 40.0, n_beams))
+    delays = np.stack([fs.focused(pos, np.array([0.06 * np.sin(a), 0.06 * np.cos(a)]), speed_of_sound=1540.0)
+                       for a in psi])
+    return params, medium, phantoms, delays
+
+
+def bench_phantoms(args) -> int:
+    """One step = the spectra (one persistent split-BF16 launch) and gated RF of every phantom and beam."""
+    import torch
+
+    from fastsimus_amd import _lib
+    from fastsimus_amd import batch as B
+    from fastsimus_amd import kernels as K
+
+    dev = torch.device("cuda", 0)
+    params, medium, phantoms, delays = phantom_workload(n_phantoms=args.n_phantoms)
+    dph = [(torch.as_tensor(p, device=dev), torch.as_tensor(r, device=dev)) for p, r in phantoms]
+    plan = B.simus_batch_precompute(dph, delays, params, medium)
+    pts, rcs, off = B._concat(dph)
+    prob = B._problem(pts, rcs, delays, None, plan, params, medium, False, dev)
+    ws = K.batch_workspace(prob, off, "mfma")
+    nf, E, T, P, S = prob.n_freq, params.n_elements, prob.n_tx, len(phantoms), int(off[-1])
+    lib = _lib.load()
+
+    def step():
+        sp = K.run_spectrum_batch(prob, off, "mfma", workspace=ws)
+        rf, _, _ = K.run_rf(sp.reshape(P * T, nf, E), plan.n_freq_full, plan.freq_idx_start, plan.n_fft,
+                            want_full=False)
+        return rf
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lib.fs_set_kernel_timing(1)
+    _lib.kernel_time_ms()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    k_ms, k_n = _lib.kernel_time_ms()
+    lib.fs_set_kernel_timing(0)
+    k_avg = k_ms / max(k_n, 1) / 1e3
+    tiles = sum(-(-int(off[q + 1] - off[q]) // 16) for q in range(P))
+    nfpad = -(-nf // 4) * 4
+    flops = tiles * nfpad * 10 * 6 * (16 * 16 * 32 * 2)  # 4 element tiles: 10 upper tile pairs
+    value = T * S * E * nf / t_step / 1e9
+
+    # plain FP32 kernels on one beam of every phantom (cost linear in beams)
+    prob1 = B._problem(pts, rcs, delays[T // 2], None, plan, params, medium, False, dev)
+    ws1 = K.batch_workspace(prob1, off, "fp32")
+    K.run_spectrum_batch(prob1, off, "fp32", workspace=ws1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    K.run_spectrum_batch(prob1, off, "fp32", workspace=ws1)
+    torch.cuda.synchronize()
+    plain_value = S * E * nf / (time.perf_counter() - t0) / 1e9
+
+    # FP64 check on a sub-batch: the first 150 scatterers of 6 phantoms, 3 beams, shared plan
+    from oracle import simus_oracle as O
+
+    sub = [(p[:150], r[:150]) for p, r in phantoms[:6]]
+    beams = delays[[0, T // 2, T - 1]]
+    rf_sub = B.simus_batch(sub, beams, params, medium, strategy="hip_mfma").rf.cpu().numpy()
+    allp = np.concatenate([p for p, _ in sub]).astype(np.float64)
+    plan_o = O.simus_plan(allp[:, 0], allp[:, 1], beams, params, medium)
+    errs = []
+    for q, (p_, r_) in enumerate(sub):
+        want = np.stack([O.simus_oracle(p_[:, 0].astype(np.float64), p_[:, 1].astype(np.float64),
+                                        r_.astype(np.float64), d, params, medium, plan_o)[0] for d in beams])
+        errs.append(rel_l2(rf_sub[q], want))
+
+    out = {
+        "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16", "precision": "split-BF16 (hi+lo, 3 products) MFMA, FP32 accumulate; FP64 r and phase reseed",
+        "data": "synthetic",
+        "config": {"workload": f"uneven phantom batch: {P} phantoms of 10^U[4,6] scatterers in a 90-degree sector "
+                               f"to 12 cm, 64-element 2.7 MHz phased array, {T} beams focused at 6 cm over "
+                               "[-40,40] deg, one persistent launch",
+                   "n_phantoms": P, "n_scat_total": S, "n_scat_min": int(np.diff(off).min()),
+                   "n_scat_max": int(np.diff(off).max()), "n_elem": E, "n_tx": T, "n_freq": nf,
+                   "n_sub": plan.n_sub, "method": "hip_mfma"},
+        "roofline": {"bound": "mfma", "achieved": flops / k_avg / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
+                     "frac": flops / k_avg / PEAK_BF16, "traffic": None, "kernel": "k_msm (persistent, all phantoms)",
+                     "kernel_ms": k_avg * 1e3, "launches": k_n},
+        "plain_fp32": {"value": plain_value, "unit": UNIT, "transmits_timed": [T // 2]},
+        "speedup_vs_plain": value / plain_value,
+        "rf_rel_l2_fp64_per_phantom": {"max": max(errs), "values": errs,
+                                       "sample": "first 150 scatterers of phantoms 0-5, beams 0, 64, 127"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_adjoint(args) -> int:
+    """Differentiable form: forward RF + gradient w.r.t. rc, x, z and c of an L2 loss, 256 x 256 jittered grid,
+    15 plane waves in [-7, 7] deg; target from rc + N(0, 0.1) at c = 1560 m/s."""
+    import torch
+
+    import fastsimus_amd as fs
+    from fastsimus_amd.autograd import simus_rf
+
+    dev = torch.device("cuda", 0)
+    params = fs.TransducerParams(freq_center=7.6e6, pitch=3.0e-4, width=2.7e-4, n_elements=128, bandwidth=0.77)
+    rng = np.random.default_rng(2028)
+    gx, gz = np.meshgrid(np.linspace(-0.02, 0.02, 256), np.linspace(0.005, 0.045, 256), indexing="ij")
+    x = (gx.ravel() + rng.uniform(-5e-5, 5e-5, gx.size)).astype(np.float32)
+    z = (gz.ravel() + rng.uniform(-5e-5, 5e-5, gz.size)).astype(np.float32)
+    rc = rng.standard_normal(x.size).astype(np.float32)
+    rc_t = (rc + 0.1 * rng.standard_normal(x.size)).astype(np.float32)
+    pos, _, _ = fs.element_positions(params.n_elements, params.pitch, params.radius)
+    D = np.stack([fs.plane_wave(pos, float(a), speed_of_sound=1540.0) for a in np.deg2rad(np.linspace(-7, 7, 15))])
+    att = 0.5
+    plan = fs.simus_precompute(np.stack([x, z], -1), rc, D, params, fs.MediumParams(attenuation=att))
+    xt, zt = torch.tensor(x, device=dev), torch.tensor(z, device=dev)
+    with torch.no_grad():
+        target = simus_rf(xt, zt, torch.tensor(rc_t, device=dev), 1560.0, D, params, plan, attenuation=att)
+
+    def step():
+        xg = xt.clone().requires_grad_(True)
+        zg = zt.clone().requires_grad_(True)
+        rg = torch.tensor(rc, device=dev, requires_grad=True)
+        cg = torch.tensor(1540.0, dtype=torch.float64, requires_grad=True)
+        rf = simus_rf(xg, zg, rg, cg, D, params, plan, attenuation=att)
+        loss = 0.5 * ((rf - target) ** 2).sum()
+        loss.backward()
+        return loss
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    nf = int(plan.selected_freqs.shape[0])
+    value = D.shape[0] * x.size * params.n_elements * nf / t_step / 1e9
+    out = {
+        "metric": "G(scatterer·element·freq)/s, forward + adjoint (dL/d rc, x, z, c)", "value": value, "unit": UNIT,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_step * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "differentiable SIMUS: 256x256 jittered grid (65536 scatterers), 15 plane waves in "
+                               "[-7,7] deg, L2 loss vs RF of rc + N(0,0.1) at c = 1560 m/s",
+                   "n_scat": int(x.size), "n_elem": params.n_elements, "n_tx": int(D.shape[0]), "n_freq": nf,
+                   "n_sub": plan.n_sub, "method": "hip (FP32) forward + hand-written adjoint"},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +288,10 @@ def main() -> int:
     ap.add_argument("--plain-tx", type=int, default=3, help="transmits timed with the plain FP32 kernel")
     ap.add_argument("--cpu-sample", type=int, default=1000, help="scatterers in the CPU-baseline / FP64 check sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("planewave75", "phantoms64", "adjoint15"), default="planewave75",
+                    help="planewave75 (default, the headline metric), phantoms64 (uneven phantom batch, phased "
+                         "array, focused beams) or adjoint15 (forward + gradient of the differentiable form)")
+    ap.add_argument("--n-phantoms", type=int, default=64)
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (calibrates the FETCH_SIZE / WRITE_SIZE "
                          "counters of a profiler against a known byte count) and exit")
@@ -142,6 +317,10 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_calib_bytes": n * 4, "launches": 2}), flush=True)
         return 0
+    if args.workload == "phantoms64":
+        return bench_phantoms(args)
+    if args.workload == "adjoint15":
+        return bench_adjoint(args)
     params, medium, x, z, rc, delays = workload(n_scat=args.n_scat, n_tx=args.n_tx)
     sc = np.stack([x, z], -1)
     plan = fs.simus_precompute(sc, rc, delays, params, medium)
