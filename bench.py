@@ -61,11 +61,11 @@ def rel_l2(a, b) -> float:
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def msm_flops_per_launch(n_scat: int, n_elem: int, n_freq: int) -> float:
-    """MFMA FLOPs issued by one k_msm launch (upper-triangular 16x16 tiles, 6 MFMAs each per 16 scatterers)."""
+def msm_flops_per_launch(n_scat: int, n_elem: int, n_freq: int, fbk: int = 4) -> float:
+    """MFMA FLOPs issued by one k_msm launch (upper-triangular 16x16 tiles, 6 MFMAs each per 16 scatterers
+    and frequency, frequencies padded to a multiple of the block depth fbk)."""
     et = 4 if n_elem <= 64 else 8
     pairs = et * (et + 1) // 2
-    fbk = 2
     nfpad = -(-n_freq // fbk) * fbk
     tiles = -(-n_scat // 16)
     return tiles * nfpad * pairs * 6 * (16 * 16 * 32 * 2)
@@ -144,6 +144,10 @@ def bench_phantoms(args) -> int:
     plan = B.simus_batch_precompute(dph, delays, params, medium)
     pts, rcs, off = B._concat(dph)
     prob = B._problem(pts, rcs, delays, None, plan, params, medium, False, dev)
+    from fastsimus_amd import tune
+
+    tuning = None if args.no_tune else tune.autotune(
+        prob, off, reps=1, candidates=[{"msm_fbk": 4, "msm_item_div": d} for d in (0, 4, 16)])
     ws = K.batch_workspace(prob, off, "mfma")
     nf, E, T, P, S = prob.n_freq, params.n_elements, prob.n_tx, len(phantoms), int(off[-1])
     lib = _lib.load()
@@ -212,6 +216,7 @@ def bench_phantoms(args) -> int:
                      "kernel_ms": k_avg * 1e3, "launches": k_n},
         "plain_fp32": {"value": plain_value, "unit": UNIT, "transmits_timed": [T // 2]},
         "speedup_vs_plain": value / plain_value,
+        "tuning": tuning,
         "rf_rel_l2_fp64_per_phantom": {"max": max(errs), "values": errs,
                                        "sample": "first 150 scatterers of phantoms 0-5, beams 0, 64, 127"},
         "cpu_baseline": None,
@@ -292,6 +297,7 @@ def main() -> int:
                     help="planewave75 (default, the headline metric), phantoms64 (uneven phantom batch, phased "
                          "array, focused beams) or adjoint15 (forward + gradient of the differentiable form)")
     ap.add_argument("--n-phantoms", type=int, default=64)
+    ap.add_argument("--no-tune", action="store_true", help="skip the tile-shape autotuner (library defaults)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (calibrates the FETCH_SIZE / WRITE_SIZE "
                          "counters of a profiler against a known byte count) and exit")
@@ -334,6 +340,9 @@ def main() -> int:
 
     prob = problem(delays)
     lib = _lib.load()
+    from fastsimus_amd import tune
+
+    tuning = tune.autotune(prob, reps=1) if not args.no_tune else None
 
     def step():
         sp = K.run_spectrum(prob, "mfma")
@@ -356,7 +365,8 @@ def main() -> int:
     evals = T * S * E * nf
     value = evals / t_step / 1e9
     k_avg = k_ms / max(k_n, 1) / 1e3
-    flops = msm_flops_per_launch(S, E, nf)
+    fbk = (tuning["chosen"].get("msm_fbk") or 4) if tuning else 4
+    flops = msm_flops_per_launch(S, E, nf, fbk)
     roof = {"bound": "mfma", "achieved": flops / k_avg / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
             "frac": flops / k_avg / PEAK_BF16, "traffic": measured_traffic(S, T, nf, E),
             "kernel": "k_msm (split-BF16 v_mfma_f32_16x16x32_bf16)", "kernel_ms": k_avg * 1e3, "launches": k_n}
@@ -407,6 +417,7 @@ def main() -> int:
                    "n_scat": S, "n_elem": E, "n_tx": T, "n_freq": nf, "n_sub": plan.n_sub, "n_fft": plan.n_fft,
                    "method": "hip_mfma"},
         "roofline": roof, "plain_fp32": plain, "speedup_vs_plain": value / plain_value, "rf_rel_l2": err,
+        "tuning": tuning,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -25,6 +25,8 @@ EXPORTS = (
     "fs_simus_spectrum",
     "fs_simus_batch_workspace_bytes",
     "fs_simus_spectrum_batch",
+    "fs_tuning_set",
+    "fs_tuning_get",
     "fs_rf_workspace_bytes",
     "fs_spectrum_to_rf",
     "fs_rf_adjoint_workspace_bytes",
@@ -55,6 +57,13 @@ class SimusDesc(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    """Mirror of ``fs_tuning`` (zero fields mean default)."""
+
+    _fields_ = [("msm_fbk", C.c_int32), ("msm_units", C.c_int32), ("msm_item_div", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
+
+
 _lib = None
 
 
@@ -80,6 +89,10 @@ def load() -> C.CDLL:
     lib.fs_simus_batch_workspace_bytes.argtypes = [D, C.c_int, C.c_int, I64]
     lib.fs_simus_spectrum_batch.restype = C.c_int
     lib.fs_simus_spectrum_batch.argtypes = [D, C.c_int, C.c_int, I64, P, P, C.c_size_t, P]
+    lib.fs_tuning_set.restype = C.c_int
+    lib.fs_tuning_set.argtypes = [D, C.c_int, C.POINTER(Tuning)]
+    lib.fs_tuning_get.restype = C.c_int
+    lib.fs_tuning_get.argtypes = [D, C.c_int, C.POINTER(Tuning)]
     lib.fs_rf_workspace_bytes.restype = C.c_size_t
     lib.fs_rf_workspace_bytes.argtypes = [C.c_int]
     lib.fs_spectrum_to_rf.restype = C.c_int

@@ -15,6 +15,8 @@ size_t mfma_workspace(const Problem& p);
 int mfma_spectrum_batch(const Problem& p, int P, const int64_t* off, cf* spect, char* ws, size_t ws_bytes,
                         hipStream_t st);
 size_t mfma_workspace_batch(const Problem& p, int P, const int64_t* off);
+fs_tuning msm_tuning(const Problem& p, int P);
+int msm_tuning_set(const Problem& p, int P, const fs_tuning* t);
 int spectrum_to_rf(const cf* spect, int NT, int NF, int E, int nff, int idx0, int N, cf* full, float* rf, float* rf_u,
                    unsigned* peak, hipStream_t st);
 size_t rf_adjoint_ws(int NT, int nkeep, int E);
@@ -184,6 +186,23 @@ int fs_simus_spectrum_batch(const fs_simus_desc* d, int method, int n_phantoms, 
     return mfma_spectrum_batch(p, n_phantoms, phantom_offsets, (cf*)spect, (char*)workspace, workspace_bytes, st);
   }
   return FS_ERR_INVALID;
+}
+
+int fs_tuning_set(const fs_simus_desc* d, int n_phantoms, const fs_tuning* t) {
+  Problem p;
+  int rc = make_problem(d, &p, false, false);
+  if (rc != FS_OK) return rc;
+  if (n_phantoms < 1 || n_phantoms > 192) return FS_ERR_INVALID;
+  return msm_tuning_set(p, n_phantoms, t);
+}
+
+int fs_tuning_get(const fs_simus_desc* d, int n_phantoms, fs_tuning* t) {
+  Problem p;
+  int rc = make_problem(d, &p, false, false);
+  if (rc != FS_OK) return rc;
+  if (!t || n_phantoms < 1 || n_phantoms > 192) return FS_ERR_INVALID;
+  *t = msm_tuning(p, n_phantoms);
+  return FS_OK;
 }
 
 size_t fs_rf_workspace_bytes(int n_tx) { return n_tx > 0 ? ((size_t)n_tx * sizeof(unsigned) + 255) / 256 * 256 : 0; }

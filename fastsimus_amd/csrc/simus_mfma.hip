@@ -38,7 +38,10 @@
 // k_msm_apply sums the chunks in a fixed order and applies every transmit
 // (E x E x n_tx per frequency, FP32).
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "fs_device.h"
@@ -131,12 +134,13 @@ __host__ __device__ constexpr bool uses_col(int c) {
 
 // Workgroup shape: one thread per (scatterer quad, element) of a tile, i.e.
 // ET waves; FBK frequencies per block, KPW per wave, WPP waves per group of KPW
-// frequencies. HEAVY variants (more than two sub-elements, or directivity at
-// every frequency) need more registers for the Green's functions and take
-// FBK = 2, KPW = 1 (half the accumulators); the others FBK = 4, KPW = 2.
-template <int ET, bool HEAVY>
+// frequencies. SHALLOW blocks (FBK = 2, KPW = 1: half the accumulators) are
+// required where the Green's functions need more registers (more than two
+// sub-elements, or directivity at every frequency) and selectable elsewhere;
+// the default is FBK = 4, KPW = 2.
+template <int ET, bool SHALLOW>
 struct MsmShape {
-  static constexpr int FBK = HEAVY ? 2 : 4;
+  static constexpr int FBK = SHALLOW ? 2 : 4;
   static constexpr int KPW = FBK / 2;           // frequencies per wave
   static constexpr int NW = ET;                 // waves per workgroup
   static constexpr int THREADS = 64 * NW;
@@ -418,10 +422,10 @@ struct WaveTiles {
 // LDS buffer. The two waves that share a SIMD run in opposite order within an
 // iteration (one multiplies while the other generates) so that matrix-core and
 // vector work overlap; one barrier per tile separates the buffers' roles.
-template <int NSUB, int ET, int MODE, int W>
+template <int NSUB, int ET, int MODE, bool SHALLOW, int W>
 __device__ __forceinline__ void msm_wave(const Problem& p, const float4* __restrict__ scat, int sb, int se, int fb,
                                          int k0, bool mfma_first, char* lds, cf* kpart) {
-  using SH = MsmShape<ET, (NSUB > 2 || MODE == MSM_FULLDIR)>;
+  using SH = MsmShape<ET, SHALLOW>;
   constexpr int FBK = SH::FBK;
   using L = FragLds<ET, FBK>;
   u32x4* const frag = (u32x4*)lds;  // buffer b at frag + b * per_buf
@@ -468,7 +472,7 @@ struct MsmPhantoms {
 // by the host before the launch) until none is left; items come largest unit
 // first. Every wave runs the same sequence of barriers: two per item, one
 // before its tile loop and one per tile.
-template <int NSUB, int ET, int MODE, int W>
+template <int NSUB, int ET, int MODE, bool SHALLOW, int W>
 __device__ __forceinline__ void msm_persistent(const Problem& p, const float4* __restrict__ scat,
                                                const MsmUnits& units, int* __restrict__ queue, int nfpad,
                                                cf* __restrict__ kpart, int k0, bool mfma_first, char* lds,
@@ -484,18 +488,18 @@ __device__ __forceinline__ void msm_persistent(const Problem& p, const float4* _
     const int fb = item - ui * units.nfb;
     const MsmUnit u = units.u[ui];
     cf* kp = kpart + (size_t)u.slot * nfpad * (ET * 16) * (ET * 16);
-    msm_wave<NSUB, ET, MODE, W>(p, scat, u.s_begin, u.s_end, fb, k0, mfma_first, lds, kp);
+    msm_wave<NSUB, ET, MODE, SHALLOW, W>(p, scat, u.s_begin, u.s_end, fb, k0, mfma_first, lds, kp);
   }
 }
 
 // ET waves per workgroup; wave w owns frequencies KPW (w / WPP) ..
 // KPW (w / WPP) + KPW - 1 of each item's block and job list w % WPP. Waves w
 // and w + 4 (and w, w + 1) take opposite orders of multiply and generate.
-template <int NSUB, int ET, int MODE>
+template <int NSUB, int ET, int MODE, bool SHALLOW>
 __global__ __launch_bounds__(64 * ET, ET == 8 ? 1 : 2) void k_msm(Problem p, const float4* __restrict__ scat,
                                                                  const MsmUnits units, int* __restrict__ queue,
                                                                  int nfpad, cf* __restrict__ kpart) {
-  using SH = MsmShape<ET, (NSUB > 2 || MODE == MSM_FULLDIR)>;
+  using SH = MsmShape<ET, SHALLOW>;
   extern __shared__ __attribute__((aligned(16))) char lds_raw[];
   __shared__ int next_item;
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -503,16 +507,16 @@ __global__ __launch_bounds__(64 * ET, ET == 8 ? 1 : 2) void k_msm(Problem p, con
   const bool mfma_first = ((w ^ (w >> 2)) & 1) == 0;
   if constexpr (SH::WPP == 4) {
     switch (w % 4) {
-      case 0: msm_persistent<NSUB, ET, MODE, 0>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
-      case 1: msm_persistent<NSUB, ET, MODE, 1>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
-      case 2: msm_persistent<NSUB, ET, MODE, 2>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
-      default: msm_persistent<NSUB, ET, MODE, 3>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
+      case 0: msm_persistent<NSUB, ET, MODE, SHALLOW, 0>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
+      case 1: msm_persistent<NSUB, ET, MODE, SHALLOW, 1>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
+      case 2: msm_persistent<NSUB, ET, MODE, SHALLOW, 2>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
+      default: msm_persistent<NSUB, ET, MODE, SHALLOW, 3>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
     }
   } else {
     if (w % 2 == 0)
-      msm_persistent<NSUB, ET, MODE, 0>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item);
+      msm_persistent<NSUB, ET, MODE, SHALLOW, 0>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item);
     else
-      msm_persistent<NSUB, ET, MODE, 1>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item);
+      msm_persistent<NSUB, ET, MODE, SHALLOW, 1>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item);
   }
 }
 
@@ -600,6 +604,7 @@ __global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, const M
 // ---------------------------------------------------------------- host side
 struct MsmPlan {
   int ET, FBK, nfb, nfpad, threads, grid, n_items;
+  bool shallow;
   size_t lds, kpart_bytes, scat_offset, queue_offset, ws_bytes;
   MsmUnits units;
   MsmPhantoms phantoms;
@@ -626,6 +631,45 @@ static int device_cus() {
   return cached[dev];
 }
 
+// ---------------------------------------------------------------- tuning cache
+// Tile-shape choices of k_msm per problem signature (device, element tiles,
+// sub-elements, Green's-function variant, frequencies, phantom count, log2 of
+// the scatterer tiles), set by an autotuner; absent entries use the defaults.
+static bool msm_fast_mode(const Problem& p) { return !p.fulldir && p.baffle == FS_BAFFLE_SOFT && p.kappa <= 2.5f; }
+
+static bool shallow_required(const Problem& p) { return p.NS > 2 || p.fulldir; }
+
+static std::array<long long, 8> tuning_key(const Problem& p, int P) {
+  long long tiles = std::max(1, (p.S + MST - 1) / MST), lg = 0;
+  while ((1LL << (lg + 1)) <= tiles) ++lg;
+  return {(long long)current_device(), p.E <= 64 ? 4 : 8, p.NS, p.fulldir ? 2 : (msm_fast_mode(p) ? 0 : 1),
+          p.NF, P, lg, 0};
+}
+
+static std::mutex tuning_mutex;
+static std::map<std::array<long long, 8>, fs_tuning> tuning_table;
+
+fs_tuning msm_tuning(const Problem& p, int P) {
+  std::lock_guard<std::mutex> lock(tuning_mutex);
+  const auto it = tuning_table.find(tuning_key(p, P));
+  if (it != tuning_table.end()) return it->second;
+  fs_tuning t{};
+  return t;
+}
+
+int msm_tuning_set(const Problem& p, int P, const fs_tuning* t) {
+  std::lock_guard<std::mutex> lock(tuning_mutex);
+  if (!t) {
+    tuning_table.erase(tuning_key(p, P));
+    return FS_OK;
+  }
+  if ((t->msm_fbk != 0 && t->msm_fbk != 2 && t->msm_fbk != 4) || t->msm_units < 0 || t->msm_units > MSM_MAX_UNITS ||
+      t->msm_item_div < 0 || t->msm_item_div > 1024)
+    return FS_ERR_INVALID;
+  tuning_table[tuning_key(p, P)] = *t;
+  return FS_OK;
+}
+
 // Scatterer chunks of a single phantom: the count whose last round of
 // equal items is fullest (rounds / chunks smallest), at least 8 tiles each.
 static int pick_chunks(int nfb, int tiles, int slots) {
@@ -644,15 +688,17 @@ static int pick_chunks(int nfb, int tiles, int slots) {
 }
 
 // Units of a batch of phantoms [off[q], off[q + 1]): one phantom is split as
-// pick_chunks says; in a batch every unit stays below 1/8 of a queue slot's
-// share of the work, so that the largest-first queue ends evenly.
-template <int ET, bool HEAVY>
-static int msm_plan_et(const Problem& p, int P, const int64_t* off, MsmPlan* out) {
-  using SH = MsmShape<ET, HEAVY>;
+// pick_chunks says (or as tuned); in a batch every unit stays below
+// 1/item_div (default 1/8) of a queue slot's share of the work, so that the
+// largest-first queue ends evenly.
+template <int ET, bool SHALLOW>
+static int msm_plan_et(const Problem& p, int P, const int64_t* off, const fs_tuning& tune, MsmPlan* out) {
+  using SH = MsmShape<ET, SHALLOW>;
   MsmPlan& m = *out;
   m = MsmPlan{};
   m.ET = ET;
   m.FBK = SH::FBK;
+  m.shallow = SHALLOW;
   m.threads = SH::THREADS;
   m.nfb = (p.NF + m.FBK - 1) / m.FBK;
   m.nfpad = m.nfb * m.FBK;
@@ -664,9 +710,11 @@ static int msm_plan_et(const Problem& p, int P, const int64_t* off, MsmPlan* out
     total += std::max(tiles[q], 1);
   }
   if (P == 1) {
-    chunks[0] = pick_chunks(m.nfb, std::max(tiles[0], 1), slots);
+    chunks[0] = tune.msm_units > 0 ? std::min(tune.msm_units, std::max(tiles[0], 1))
+                                   : pick_chunks(m.nfb, std::max(tiles[0], 1), slots);
   } else {
-    double tmax = std::max(8.0, (double)total * m.nfb / (slots * 8.0));
+    const double div = tune.msm_item_div > 0 ? (double)tune.msm_item_div : 8.0;
+    double tmax = std::max(8.0, (double)total * m.nfb / (slots * div));
     for (;;) {
       int n = 0;
       for (int q = 0; q < P; ++q) {
@@ -709,9 +757,11 @@ static int msm_plan(const Problem& p, int P, const int64_t* off, MsmPlan* m) {
   if (off[0] != 0 || off[P] != p.S) return FS_ERR_INVALID;
   for (int q = 0; q < P; ++q)
     if (off[q + 1] < off[q]) return FS_ERR_INVALID;
-  const bool heavy = p.NS > 2 || p.fulldir;
-  if (p.E <= 64) return heavy ? msm_plan_et<4, true>(p, P, off, m) : msm_plan_et<4, false>(p, P, off, m);
-  return heavy ? msm_plan_et<8, true>(p, P, off, m) : msm_plan_et<8, false>(p, P, off, m);
+  const fs_tuning tune = msm_tuning(p, P);
+  const bool shallow = shallow_required(p) || tune.msm_fbk == 2;
+  if (p.E <= 64)
+    return shallow ? msm_plan_et<4, true>(p, P, off, tune, m) : msm_plan_et<4, false>(p, P, off, tune, m);
+  return shallow ? msm_plan_et<8, true>(p, P, off, tune, m) : msm_plan_et<8, false>(p, P, off, tune, m);
 }
 
 size_t mfma_workspace_batch(const Problem& p, int P, const int64_t* off) {
@@ -726,10 +776,10 @@ size_t mfma_workspace(const Problem& p) {
   return mfma_workspace_batch(p, 1, off);
 }
 
-template <int NSUB, int ET, int MODE>
+template <int NSUB, int ET, int MODE, bool SHALLOW>
 static int launch_msm(const Problem& p, const MsmPlan& m, const float4* scat, int* queue, cf* kpart,
                       hipStream_t st) {
-  auto kern = k_msm<NSUB, ET, MODE>;
+  auto kern = k_msm<NSUB, ET, MODE, SHALLOW>;
   static bool lds_set[MAX_DEVICES] = {false};  // dynamic LDS above 64 KiB: set once per device
   const int dev = current_device();
   if (!lds_set[dev]) {
@@ -747,10 +797,18 @@ static int launch_msm(const Problem& p, const MsmPlan& m, const float4* scat, in
 template <int NSUB, int ET>
 static int launch_msm_fd(const Problem& p, const MsmPlan& m, const float4* scat, int* queue, cf* kpart,
                          hipStream_t st) {
-  if (p.fulldir) return launch_msm<NSUB, ET, MSM_FULLDIR>(p, m, scat, queue, kpart, st);
-  if (p.baffle == FS_BAFFLE_SOFT && p.kappa <= 2.5f)
-    return launch_msm<NSUB, ET, MSM_FAST>(p, m, scat, queue, kpart, st);
-  return launch_msm<NSUB, ET, MSM_GENERIC>(p, m, scat, queue, kpart, st);
+  if constexpr (NSUB > 2) {
+    if (p.fulldir) return launch_msm<NSUB, ET, MSM_FULLDIR, true>(p, m, scat, queue, kpart, st);
+    if (msm_fast_mode(p)) return launch_msm<NSUB, ET, MSM_FAST, true>(p, m, scat, queue, kpart, st);
+    return launch_msm<NSUB, ET, MSM_GENERIC, true>(p, m, scat, queue, kpart, st);
+  } else {
+    if (p.fulldir) return launch_msm<NSUB, ET, MSM_FULLDIR, true>(p, m, scat, queue, kpart, st);
+    if (msm_fast_mode(p))
+      return m.shallow ? launch_msm<NSUB, ET, MSM_FAST, true>(p, m, scat, queue, kpart, st)
+                       : launch_msm<NSUB, ET, MSM_FAST, false>(p, m, scat, queue, kpart, st);
+    return m.shallow ? launch_msm<NSUB, ET, MSM_GENERIC, true>(p, m, scat, queue, kpart, st)
+                     : launch_msm<NSUB, ET, MSM_GENERIC, false>(p, m, scat, queue, kpart, st);
+  }
 }
 
 template <int NSUB>

@@ -114,6 +114,23 @@ size_t fs_simus_batch_workspace_bytes(const fs_simus_desc* d, int method, int n_
 int fs_simus_spectrum_batch(const fs_simus_desc* d, int method, int n_phantoms, const int64_t* phantom_offsets,
                             float* spect, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Tile-shape choices of the batched method's multistatic-matrix kernel for one problem signature
+ * (device, element count class, sub-elements, directivity/baffle variant, frequencies, phantom count,
+ * log2 of the scatterer tiles). Zero fields mean "default". Stored in a mutex-guarded process-wide
+ * cache that fs_simus_spectrum / fs_simus_spectrum_batch consult; set by an autotuner
+ * (fastsimus_amd.tune). No counterpart in the reference (its CUDA constants are fixed,
+ * src/fast_simus/kernels/cuda_simus.py:39-45). */
+typedef struct fs_tuning {
+  int32_t msm_fbk;       /* frequencies per block: 0 (default 4), 2 or 4; 2 is forced where registers need it */
+  int32_t msm_units;     /* single phantom: scatterer chunks (0 = chosen from the CU count) */
+  int32_t msm_item_div;  /* batches: largest work item <= 1/msm_item_div of a queue slot's share (0 = 8) */
+  int32_t reserved[5];
+} fs_tuning;
+/* t == NULL forgets the entry. n_phantoms >= 1 (1 for fs_simus_spectrum). */
+int fs_tuning_set(const fs_simus_desc* d, int n_phantoms, const fs_tuning* t);
+/* Entry the next call with this signature will use (zeros if none). */
+int fs_tuning_get(const fs_simus_desc* d, int n_phantoms, fs_tuning* t);
+
 /* Place the band into the full spectrum and compute gated RF per transmit:
  *   spectrum_full[t][i][e] (complex64, i < n_ff), zero outside [idx0, idx0 + n_freq);
  *   rf[t][n][e] = gate(irfft(conj(full), n_fft)[n]), n < (n_fft + 1) / 2.
