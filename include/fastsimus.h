@@ -22,6 +22,9 @@
  *   fs_pfield             <- pfield_metal / _freq_outer_python
  *                            (src/fast_simus/kernels/metal_pfield.py:89-219,
  *                             src/fast_simus/_pfield_strategies.py:53-88)
+ *   fs_simus_spectrum_batch <- a loop of simus_cuda calls over phantoms (no batched API in the
+ *                            reference; its harness loops, tests/_picmus_phantom_shared.py:319-367)
+ *   fs_tuning_set / get   <- the fixed tuning constants of kernels/cuda_simus.py:39-45
  */
 #ifndef FASTSIMUS_H
 #define FASTSIMUS_H
