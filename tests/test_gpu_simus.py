@@ -242,3 +242,23 @@ def test_pfield_convex_out_of_field(cuda):
     want = O.pfield_oracle(g[:, 0], g[:, 1], np.zeros(128), p, fs.MediumParams(), pl)
     assert pf[3] == 0.0 and pf[1] == 0.0  # behind the array / inside the arc
     assert rel_l2(pf, want) < FP32_TOL
+
+
+# ----------------------------------------------------------------- element counts that do not fill 16-element tiles
+@pytest.mark.parametrize("n_elem", [1, 7, 33, 100, 192])
+def test_ragged_element_counts(n_elem, cuda):
+    """Partial element tiles of both methods; the batched method stops at 128 elements (NotImplementedError)."""
+    p = fs.TransducerParams(freq_center=5e6, pitch=3.0e-4, width=2.5e-4, n_elements=n_elem, bandwidth=0.7)
+    m = fs.MediumParams(attenuation=0.3)
+    x, z, rc = speckle(90 + n_elem, 23, xlim=(-0.01, 0.01), zlim=(0.005, 0.03))
+    D = np.stack([fs.plane_wave(_pos(p), np.deg2rad(a), speed_of_sound=1540.0) for a in (-4.0, 9.0)])
+    want = _oracle_rf(x, z, rc, D, p, m)
+    print("n_elem", n_elem, flush=True)
+    rf32, _ = _run("hip", x, z, rc, D, p, m)
+    assert rel_l2(rf32, want) < FP32_TOL
+    if n_elem > 128:
+        with pytest.raises(NotImplementedError):
+            _run("hip_mfma", x, z, rc, D, p, m)
+    else:
+        rfm, _ = _run("hip_mfma", x, z, rc, D, p, m)
+        assert rel_l2(rfm, want) < MFMA_TOL
