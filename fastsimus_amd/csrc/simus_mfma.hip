@@ -440,7 +440,13 @@ __device__ __forceinline__ void msm_wave(const Problem& p, const float4* __restr
     const bool more = s0 + MST < se;
     const int nb = buf ^ 1;
     if (mfma_first) t.step(frag + buf * L::per_buf, k0);
-    if (more) gen_fragments<NSUB, FBK, MODE, ET>(p, ed, bs, scat, s0 + MST, se, frag + nb * L::per_buf);
+    if (more) {
+      // The generate-first wave of a SIMD gets the vector units first, so that its
+      // multiply phase starts while the other wave still has vector work to overlap.
+      if (!mfma_first) __builtin_amdgcn_s_setprio(1);
+      gen_fragments<NSUB, FBK, MODE, ET>(p, ed, bs, scat, s0 + MST, se, frag + nb * L::per_buf);
+      __builtin_amdgcn_s_setprio(0);
+    }
     if (!mfma_first) t.step(frag + buf * L::per_buf, k0);
     __syncthreads();
     buf = nb;
