@@ -149,11 +149,12 @@ struct MsmShape {
 };
 
 // Green's-function variants of k_msm: FAST = soft baffle, centre-frequency
-// directivity with |kappa sin(theta)| <= 2.5 (branch-free; distances below
-// 1e-30 m are taken as 1e-30 m, which only moves the angle of a scatterer
-// sitting on a sub-element centroid, whose soft-baffle weight is ~1e-16 either
-// way); GENERIC = any baffle, centre-frequency directivity; FULLDIR =
-// directivity at every frequency.
+// directivity with |kappa sin(theta)| <= 2.5 (branch-free); GENERIC = any
+// baffle, centre-frequency directivity; FULLDIR = directivity at every
+// frequency. Distances below 1e-15 m are taken as 1e-15 m (both are clipped to
+// c / (2 fc) for the amplitude and phase), which only moves the angle of a
+// scatterer sitting on a sub-element centroid; GENERIC and FULLDIR handle an
+// exact zero distance as the reference does (angle pi/2).
 enum MsmMode { MSM_FAST = 0, MSM_GENERIC = 1, MSM_FULLDIR = 2 };
 
 // sin(a)/a for |a| <= 2.5: Taylor series to a^14 (truncation < 5e-9 on that range).
@@ -275,16 +276,13 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<N
       const double dz = zd - szj;
       double r2 = fma(dx, dx, dz * dz);
       bool zero = false;
-      if constexpr (MODE == MSM_FAST) {
-        r2 = fmax(r2, 1e-60);
-      } else {
-        zero = !(r2 > 0.0);
-        r2 = zero ? 1.0 : r2;
-      }
-      double y = __builtin_amdgcn_rsq(r2);
-      y = y * fma(-0.5 * r2, y * y, 1.5);  // Newton step on 1/sqrt
+      if constexpr (MODE != MSM_FAST) zero = !(r2 > 0.0);
+      r2 = fmax(r2, 1e-30);  // keeps the FP32 estimate below finite
+      // 1/r: FP32 hardware estimate, one FP64 Newton step (relative error ~1e-14)
+      const float rinv = __builtin_amdgcn_rsqf((float)r2);
+      double y = (double)rinv;
+      y = y * fma(-0.5 * r2, y * y, 1.5);
       const double rcl = fmax(zero ? 0.0 : r2 * y, p.min_dist);
-      const float rinv = (float)y;
       const float sa = zero ? 1.0f : (float)dx * rinv;
       const float ca = zero ? 0.0f : fabsf((float)dz) * rinv;
       const float st = sa * ed.ce - ca * ed.se;
