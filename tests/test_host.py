@@ -206,6 +206,8 @@ def test_library_rejects_bad_descriptors():
     assert lib.fs_spectrum_to_rf(None, 1, 1, 1, 1, 0, 4, None, None, None, None, 0, None) == _lib.FS_ERR_INVALID
     assert lib.fs_pfield(None, None, None, 0, None) == _lib.FS_ERR_INVALID
     assert lib.fs_simus_adjoint(None, None, None, None, None, None, None, 0, None) == _lib.FS_ERR_INVALID
+    assert lib.fs_simus_spectrum_batch(None, 1, 1, None, None, None, 0, None) == _lib.FS_ERR_INVALID
+    assert lib.fs_tuning_get(None, 1, None) == _lib.FS_ERR_INVALID
     with pytest.raises(ValueError):
         _lib.check(_lib.FS_ERR_INVALID, "x")
     with pytest.raises(NotImplementedError):
@@ -221,3 +223,11 @@ def test_compute_fails_loudly_without_gpu():
     p = fs.L11_5v()
     with pytest.raises(RuntimeError, match="GPU"):
         fs.simus(np.array([[0.0, 0.02]]), np.ones(1), np.zeros(128), p)
+
+
+def test_missing_library_raises_instead_of_falling_back(monkeypatch, tmp_path):
+    """The product path has no CPU route: without libfastsimus.so every compute call raises."""
+    monkeypatch.setattr(_lib, "LIB_PATH", tmp_path / "libfastsimus.so")
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(RuntimeError, match="not built"):
+        _lib.load()
