@@ -198,21 +198,37 @@ __global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_
 }
 
 // spect[tx][k][e] = corr * probe(k) * sum_w part[w][k][e]   (k < NF)
-__global__ void k_reduce(Problem p, int nbuf, int nparts, int k_begin, int k_end, const cf* __restrict__ part,
-                         const float* __restrict__ pr, int tx, cf* __restrict__ spect) {
-  const long n = (long)(k_end - k_begin) * p.E;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const int k = k_begin + (int)(i / p.E);
-    const int e = (int)(i % p.E);
-    if (k >= p.NF) continue;
-    float sx = 0.f, sy = 0.f;
-    for (int w = 0; w < nparts; ++w) {
+// spect[tx][k][e] = corr probe_k sum_w part[w][k - k_begin][e]. Block (64, RG): 64 consecutive
+// elements of one frequency; thread row y sums parts y, y + RG, ... and the RG row sums are added
+// in a fixed order, so the result does not depend on scheduling.
+constexpr int RG = 4;
+
+__global__ __launch_bounds__(64 * RG) void k_reduce(Problem p, int nbuf, int nparts, int k_begin,
+                                                    const cf* __restrict__ part, const float* __restrict__ pr,
+                                                    int tx, cf* __restrict__ spect) {
+  __shared__ float2 rows[RG][64];
+  const int e = blockIdx.x * 64 + threadIdx.x;
+  const int k = k_begin + blockIdx.y;
+  const bool ok = e < p.E && k < p.NF;
+  float sx = 0.f, sy = 0.f;
+  if (ok) {
+    for (int w = threadIdx.y; w < nparts; w += RG) {
       const cf v = part[((size_t)w * nbuf + (k - k_begin)) * p.E + e];
       sx += v.x;
       sy += v.y;
     }
-    const float s = p.corr * pr[k];
-    spect[((size_t)tx * p.NF + k) * p.E + e] = {sx * s, sy * s};
+  }
+  rows[threadIdx.y][threadIdx.x] = make_float2(sx, sy);
+  __syncthreads();
+  if (threadIdx.y == 0 && ok) {
+    float tx_ = 0.f, ty_ = 0.f;
+#pragma unroll
+    for (int r = 0; r < RG; ++r) {
+      tx_ += rows[r][threadIdx.x].x;
+      ty_ += rows[r][threadIdx.x].y;
+    }
+    const float sc = p.corr * pr[k];
+    spect[((size_t)tx * p.NF + k) * p.E + e] = {tx_ * sc, ty_ * sc};
   }
 }
 
@@ -228,9 +244,10 @@ Fp32Plan fp32_plan(const Problem& p) {
   Fp32Plan P{};
   P.nfb = (p.NF + FB - 1) / FB;
   P.nfp = P.nfb * FB;
-  // Q buffer holds fbg frequency blocks for all scatterers: cap it near 256 MiB.
+  // Q buffer holds fbg frequency blocks for all scatterers: cap it near 2 GiB (of 288 GB of HBM), so
+  // that large problems still get several blocks per group and few scatterer partitions to reduce.
   const size_t per_block = (size_t)p.S * FB * sizeof(cf);
-  P.fbg = (int)std::max<size_t>(1, std::min<size_t>((size_t)P.nfb, (256u << 20) / std::max<size_t>(per_block, 1)));
+  P.fbg = (int)std::max<size_t>(1, std::min<size_t>((size_t)P.nfb, ((size_t)2 << 30) / std::max<size_t>(per_block, 1)));
   const int etiles = (p.E + 63) / 64;
   // enough waves per RX launch to fill 256 CUs several times over
   const long want_waves = 8192;
@@ -305,10 +322,9 @@ int fp32_spectrum(const Problem& p, cf* spect, char* ws, hipStream_t st) {
         default: return FS_ERR_UNSUPPORTED;
       }
       if (rc != FS_OK) return rc;
-      const int kb = fb0 * FB, ke = (fb0 + nb) * FB;
-      const long n = (long)(ke - kb) * p.E;
-      const int blocks = (int)std::min<long>((n + 255) / 256, 65536);
-      hipLaunchKernelGGL(k_reduce, dim3(blocks), dim3(256), 0, st, p, P.fbg * FB, P.nparts, kb, ke, part, pr, tx, spect);
+      const int kb = fb0 * FB, ke = std::min((fb0 + nb) * FB, p.NF);
+      hipLaunchKernelGGL(k_reduce, dim3((p.E + 63) / 64, ke - kb), dim3(64, RG), 0, st, p, P.fbg * FB, P.nparts, kb,
+                         part, pr, tx, spect);
     }
   }
   return hipGetLastError() == hipSuccess ? FS_OK : FS_ERR_LAUNCH;
