@@ -26,12 +26,18 @@ from .plan import SimusPlan
 
 
 class _State:
-    def __init__(self, params, plan, delays, apod, attenuation):
+    def __init__(self, params, plan, delays, apod, attenuation, method):
         self.params = params
         self.plan = plan
         self.delays = delays
         self.apod = apod
         self.attenuation = attenuation
+        self.method = method
+
+
+# From this many transmits on, the multistatic-matrix (MFMA) forward, whose cost does not grow
+# with the transmit count, is faster than the per-transmit FP32 kernels.
+MFMA_MIN_TX = 3
 
 
 def _problem(st: _State, x, z, rc, c: float):
@@ -48,7 +54,7 @@ class _SimusRF(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, z, rc, c, st: _State):
         prob = _problem(st, x, z, rc, float(c))
-        spect = K.run_spectrum(prob, "fp32")
+        spect = K.run_spectrum(prob, st.method)
         rf, _, rf_u = K.run_rf(spect, st.plan.n_freq_full, st.plan.freq_idx_start, st.plan.n_fft, want_full=False,
                                want_ungated=True)
         ctx.st = st
@@ -89,12 +95,23 @@ class _SimusRF(torch.autograd.Function):
 
 
 def simus_rf(x: torch.Tensor, z: torch.Tensor, rc: torch.Tensor, speed_of_sound, delays, params: TransducerParams,
-             plan: SimusPlan, *, attenuation: float = 0.0, tx_apodization=None) -> torch.Tensor:
-    """Gated RF ``(n_tx, n_samples, n_elements)``, differentiable in x, z, rc and speed_of_sound."""
+             plan: SimusPlan, *, attenuation: float = 0.0, tx_apodization=None, method: str = "auto") -> torch.Tensor:
+    """Gated RF ``(n_tx, n_samples, n_elements)``, differentiable in x, z, rc and speed_of_sound.
+
+    ``method`` selects the forward: ``"fp32"`` (per-transmit FP32 kernels), ``"mfma"`` (split-BF16
+    multistatic matrix, at most 128 elements) or ``"auto"`` (``"mfma"`` from ``MFMA_MIN_TX``
+    transmits on where it applies). The backward is the same hand-written adjoint either way.
+    """
     if not isinstance(speed_of_sound, torch.Tensor):
         speed_of_sound = torch.tensor(float(speed_of_sound), dtype=torch.float64)
     dev = K.hip_device(x, z, rc)
     x = x.to(device=dev, dtype=torch.float32)
     z = z.to(device=dev, dtype=torch.float32)
     rc = rc.to(device=dev, dtype=torch.float32)
-    return _SimusRF.apply(x, z, rc, speed_of_sound, _State(params, plan, delays, tx_apodization, attenuation))
+    if method == "auto":
+        n_tx = 1 if (delays.dim() if isinstance(delays, torch.Tensor) else len(getattr(delays, "shape", ()))) < 2 \
+            else int(delays.shape[0])
+        method = "mfma" if (n_tx >= MFMA_MIN_TX and params.n_elements <= 128) else "fp32"
+    if method not in ("fp32", "mfma"):
+        raise ValueError(f"unknown method {method!r}")
+    return _SimusRF.apply(x, z, rc, speed_of_sound, _State(params, plan, delays, tx_apodization, attenuation, method))

@@ -239,6 +239,11 @@ __global__ __launch_bounds__(256) void k_adj_pos(Problem p, int tx, int nfp, con
     }
     const cf* TAe = TAt + (size_t)e * nfp;
     const cf* TGe = TGt + (size_t)e * nfp;
+    // dg/dth = E (a + b f) is linear in f: accumulate S0 = sum_k E W and S1 = sum_k f E W over the
+    // block, then Re(dg W) summed over k = a Re(S0) + Re(b S1).
+    cf S0[NSUB], S1[NSUB];
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) S0[j] = S1[j] = {0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < AFB; ++k) {
       const float f = (float)(fstart + (double)k * p.df);
@@ -247,13 +252,19 @@ __global__ __launch_bounds__(256) void k_adj_pos(Problem p, int tx, int nfp, con
       cmac(W, P[k], TGe[k]);
 #pragma unroll
       for (int j = 0; j < NSUB; ++j) {
-        // dg/dth = E (a + b f); Re(dg W)
         const cf EW = cmul(E[j], W);
-        gx += dc[j].ax * EW.x + f * (dc[j].bx.x * EW.x - dc[j].bx.y * EW.y);
-        gz += dc[j].az * EW.x + f * (dc[j].bz.x * EW.x - dc[j].bz.y * EW.y);
-        gc += dc[j].ac * EW.x + f * (dc[j].bc.x * EW.x - dc[j].bc.y * EW.y);
+        S0[j].x += EW.x;
+        S0[j].y += EW.y;
+        S1[j].x = fmaf(f, EW.x, S1[j].x);
+        S1[j].y = fmaf(f, EW.y, S1[j].y);
         E[j] = cmul(E[j], q[j]);
       }
+    }
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+      gx += dc[j].ax * S0[j].x + (dc[j].bx.x * S1[j].x - dc[j].bx.y * S1[j].y);
+      gz += dc[j].az * S0[j].x + (dc[j].bz.x * S1[j].x - dc[j].bz.y * S1[j].y);
+      gc += dc[j].ac * S0[j].x + (dc[j].bc.x * S1[j].x - dc[j].bc.y * S1[j].y);
     }
   }
   if (!valid) return;
@@ -272,18 +283,37 @@ __global__ void k_sum_parts(const float* __restrict__ part, int nb, int S, int a
   out[s] = accumulate ? out[s] + a : a;
 }
 
-// d_c = sum over all (t, b, s) of dc partials, in double, one workgroup
-__global__ void k_sum_dc(const float* __restrict__ part, long n, int accumulate, double* __restrict__ out) {
-  __shared__ double red[256];
-  double a = 0.0;
-  for (long i = threadIdx.x; i < n; i += 256) a += (double)part[i];
+// d_c = sum over all (t, b, s) of the dc partials, in double, in two fixed-order stages:
+// workgroup g sums the segment [g L, (g + 1) L) into seg[g], then one workgroup sums seg.
+constexpr int DC_SEGMENTS = 1024;
+
+__device__ __forceinline__ double block_sum_256(double a, double* red) {
   red[threadIdx.x] = a;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = accumulate ? out[0] + red[0] : red[0];
+  return red[0];
+}
+
+__global__ __launch_bounds__(256) void k_sum_dc_segments(const float* __restrict__ part, long n,
+                                                         double* __restrict__ seg) {
+  __shared__ double red[256];
+  const long len = (n + DC_SEGMENTS - 1) / DC_SEGMENTS;
+  const long b = (long)blockIdx.x * len, e = min(n, b + len);
+  double a = 0.0;
+  for (long i = b + threadIdx.x; i < e; i += 256) a += (double)part[i];
+  const double t = block_sum_256(a, red);
+  if (threadIdx.x == 0) seg[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void k_sum_dc(const double* __restrict__ seg, double* __restrict__ out) {
+  __shared__ double red[256];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < DC_SEGMENTS; i += 256) a += seg[i];
+  const double t = block_sum_256(a, red);
+  if (threadIdx.x == 0) out[0] = t;
 }
 
 __global__ void k_ppc(Problem p, int nfp, cf* __restrict__ ppc) {
@@ -304,7 +334,7 @@ void launch_tx_table(const Problem& p, int nfp, cf* A, hipStream_t st);  // simu
 
 struct AdjLayout {
   int nfb, nfp;
-  size_t oA, oTA, oTG, oGc, oppc, oP, oB, odrc, odx, odz, odc, total;
+  size_t oA, oTA, oTG, oGc, oppc, oP, oB, odrc, odx, odz, odc, oseg, total;
 };
 
 static size_t al(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -328,6 +358,7 @@ static AdjLayout adj_layout(const Problem& p) {
   L.odx = o;  o = al(o + part);
   L.odz = o;  o = al(o + part);
   L.odc = o;  o = al(o + part * p.NT);
+  L.oseg = o; o = al(o + DC_SEGMENTS * sizeof(double));
   L.total = o;
   return L;
 }
@@ -378,8 +409,9 @@ int adjoint_fp32(const Problem& p, const cf* dspect, float* d_rc, float* d_x, fl
       default: return FS_ERR_UNSUPPORTED;
     }
   }
-  hipLaunchKernelGGL(k_sum_dc, dim3(1), dim3(256), 0, st, (const float*)(ws + L.odc), (long)p.NT * L.nfb * p.S, 0,
-                     d_c);
+  hipLaunchKernelGGL(k_sum_dc_segments, dim3(DC_SEGMENTS), dim3(256), 0, st, (const float*)(ws + L.odc),
+                     (long)p.NT * L.nfb * p.S, (double*)(ws + L.oseg));
+  hipLaunchKernelGGL(k_sum_dc, dim3(1), dim3(256), 0, st, (const double*)(ws + L.oseg), d_c);
   return hipGetLastError() == hipSuccess ? FS_OK : FS_ERR_LAUNCH;
 }
 

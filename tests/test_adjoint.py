@@ -88,7 +88,8 @@ def test_torch_oracle_gradient_matches_finite_differences():
 
 
 @pytest.mark.gpu
-def test_hip_adjoint_matches_fp64_gradient(cuda):
+@pytest.mark.parametrize("method", ["fp32", "mfma"])
+def test_hip_adjoint_matches_fp64_gradient(method, cuda):
     from fastsimus_amd.autograd import simus_rf
 
     x, z, rc, rc_t, D = _scene(10)
@@ -102,7 +103,7 @@ def test_hip_adjoint_matches_fp64_gradient(cuda):
     zt = torch.tensor(z, dtype=torch.float32, device=cuda, requires_grad=True)
     rt = torch.tensor(rc, dtype=torch.float32, device=cuda, requires_grad=True)
     ct = torch.tensor(1540.0, dtype=torch.float64, requires_grad=True)
-    rf = simus_rf(xt, zt, rt, ct, D, PARAMS, plan, attenuation=ATT)
+    rf = simus_rf(xt, zt, rt, ct, D, PARAMS, plan, attenuation=ATT, method=method)
     tgt = target.to(torch.float32).to(cuda)
     loss = 0.5 * ((rf - tgt) ** 2).sum()
     loss.backward()
