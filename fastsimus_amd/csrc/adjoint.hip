@@ -14,7 +14,7 @@
 //   g_j(f)      = amp E_j(f),  E_j(f) = exp(f r_c (-alpha' + i 2 pi / c))
 //   dg_j/dth    = E_j(f) (a_th + b_th f)
 // with real a_th (obliquity, directivity and 1/sqrt(r_c) terms) and complex
-// b_th (propagation terms); see coeffs() below.
+// b_th (propagation terms); see coeffs_fast() below.
 //
 // k_adj_tx (lane = scatterer): P_t, B_t for one transmit, frequency blocks of
 // 32, stored to HBM; the rc gradient is reduced on the fly.
@@ -34,18 +34,27 @@ struct Dcoef {
   cf bx, bz, bc;     // per-Hz parts
 };
 
-// amp, propagation and their derivatives for one (s, e, j) term.
-__device__ __forceinline__ void coeffs(const Problem& p, double xs, double zs, int e, int j, float fc_hz, Geo& geo,
-                                       Dcoef& dc) {
+// amp, propagation and their derivatives for one (s, e, j) term, with the gfx950 hardware
+// transcendentals (1/r from the FP32 estimate plus one FP64 Newton step, v_sin / v_cos in
+// revolutions, reciprocals) and the propagation constants of GeoF (attenuation pre-multiplied
+// by log2 e).
+__device__ __forceinline__ void coeffs_fast(const Problem& p, double xs, double zs, int e, int j, float fc_hz,
+                                            GeoF& geo, Dcoef& dc) {
   const int idx = e * p.NS + j;
   const double dx = xs - p.sub_x[idx];
   const double dz = zs - p.sub_z[idx];
-  const double r = sqrt(fma(dx, dx, dz * dz));
+  double r2 = fma(dx, dx, dz * dz);
+  const bool zero = !(r2 > 0.0);
+  r2 = fmax(r2, 1e-30);
+  const float y32 = __builtin_amdgcn_rsqf((float)r2);
+  double y = (double)y32;
+  y = y * fma(-0.5 * r2, y * y, 1.5);
+  const double r = zero ? 0.0 : r2 * y;
   const bool clipped = r < p.min_dist;
   const double rcl = clipped ? p.min_dist : r;
-  const float rinv = 1.0f / (float)(r + 1e-16);
-  const float sa = (float)(dx + 1e-16) * rinv;
-  const float ca = fabsf((float)dz) * rinv;
+  const float rinv = (float)y;
+  const float sa = zero ? 1.0f : (float)dx * rinv;
+  const float ca = zero ? 0.0f : fabsf((float)dz) * rinv;
   const float ce = (float)p.ecos[e], se = (float)p.esin[e];
   const float st = sa * ce - ca * se;
   const float ct = ca * ce + sa * se;
@@ -58,53 +67,48 @@ __device__ __forceinline__ void coeffs(const Problem& p, double xs, double zs, i
     obl = 1.0f;
     dobl = 0.0f;
   } else {
-    const float den = ct + p.baffle_b;
-    obl = ct / den;
-    dobl = -st * p.baffle_b / (den * den);
+    const float iden = __builtin_amdgcn_rcpf(ct + p.baffle_b);
+    obl = ct * iden;
+    dobl = -st * p.baffle_b * iden * iden;
   }
   if (horizon) {
     obl = 1e-16f;
     dobl = 0.0f;
   }
-  // centre-frequency directivity D = sin(y)/y, y = kappa sin(theta)
-  const float y = p.kappa * st;
+  const float yk = p.kappa * st;
   float D, dDdy;
-  if (fabsf(y) < 1e-3f) {
-    D = 1.0f - y * y / 6.0f;
-    dDdy = -y / 3.0f;
+  if (fabsf(yk) < 1e-3f) {
+    D = 1.0f - yk * yk / 6.0f;
+    dDdy = -yk / 3.0f;
   } else {
-    float sy, cy;
-    sincosf(y, &sy, &cy);
-    D = sy / y;
-    dDdy = (cy - D) / y;
+    const float rev = yk * (0.5f / kPi);
+    const float iy = __builtin_amdgcn_rcpf(yk);
+    D = __builtin_amdgcn_sinf(rev) * iy;
+    dDdy = (__builtin_amdgcn_cosf(rev) - D) * iy;
   }
   const float rcf = (float)rcl;
-  const float rs = rsqrtf(rcf) * p.inv_nsub;
+  const float rs = __builtin_amdgcn_rsqf(rcf) * p.inv_nsub;
   const float amp = obl * D * rs;
-  // theta derivatives: dth/dx = |dz| / r^2, dth/dz = -dx sgn(dz) / r^2
   const float r2inv = rinv * rinv;
   const float thx = fabsf((float)dz) * r2inv;
   const float thz = -(float)dx * (dz >= 0.0 ? 1.0f : -1.0f) * r2inv;
   const float amp_th = (dobl * D + obl * dDdy * p.kappa * ct) * rs;
-  // r_c derivatives
   const float rhox = clipped ? 0.0f : sa;
   const float rhoz = clipped ? 0.0f : (float)dz * rinv;
-  const float rhoc = clipped ? 0.5f / fc_hz : 0.0f;
-  const float half = -0.5f / rcf;  // d ln(r_c^-1/2) / d r_c
-  // kappa_p = -alpha' + i 2 pi / c  (per metre per Hz)
+  const float rhoc = clipped ? 0.5f * __builtin_amdgcn_rcpf(fc_hz) : 0.0f;
+  const float half = -0.5f * __builtin_amdgcn_rcpf(rcf);
   const float inv_c = (float)p.inv_c;
   const cf kp = {-p.att_coef, 2.0f * kPi * inv_c};
   dc.ax = amp_th * thx + amp * half * rhox;
   dc.az = amp_th * thz + amp * half * rhoz;
   dc.bx = {amp * kp.x * rhox, amp * kp.y * rhox};
   dc.bz = {amp * kp.x * rhoz, amp * kp.y * rhoz};
-  // c: phase 2 pi f r_c / c, the clip c / (2 fc) and kappa = pi fc seg / c
   const float dD_dc = dDdy * st * (-p.kappa * inv_c);
   dc.ac = obl * dD_dc * rs + amp * half * rhoc;
   dc.bc = {amp * kp.x * rhoc, amp * (kp.y * rhoc - 2.0f * kPi * rcf * inv_c * inv_c)};
   geo.t = rcl * p.inv_c;
   geo.amp = amp;
-  geo.att = p.att_coef * rcf;
+  geo.att = p.att_coef * rcf * kLog2e;
   geo.sint = st;
 }
 
@@ -163,9 +167,9 @@ __global__ __launch_bounds__(256) void k_adj_tx(Problem p, int tx, int nfp, cons
     cf g[NSUB], q[NSUB];
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
-      const Geo geo = geometry(p, xs, zs, e, j);
-      g[j] = green_at(geo, fstart);
-      q[j] = green_step(geo, p.df);
+      const GeoF geo = geometry_fast(p, xs, zs, e, j);
+      g[j] = green_fast(geo, fstart, geo.amp);
+      q[j] = green_step_fast(geo, p.df);
     }
     const cf* Ae = At + (size_t)e * nfp;
     const cf* Ge = Gt + (size_t)e * nfp;
@@ -231,11 +235,10 @@ __global__ __launch_bounds__(256) void k_adj_pos(Problem p, int tx, int nfp, con
     Dcoef dc[NSUB];
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
-      Geo geo;
-      coeffs(p, xs, zs, e, j, fc_hz, geo, dc[j]);
-      geo.amp = 1.0f;  // E_j carries the propagation factor only
-      E[j] = green_at(geo, fstart);
-      q[j] = green_step(geo, p.df);
+      GeoF geo;
+      coeffs_fast(p, xs, zs, e, j, fc_hz, geo, dc[j]);
+      E[j] = green_fast(geo, fstart, 1.0f);  // E_j carries the propagation factor only
+      q[j] = green_step_fast(geo, p.df);
     }
     const cf* TAe = TAt + (size_t)e * nfp;
     const cf* TGe = TGt + (size_t)e * nfp;

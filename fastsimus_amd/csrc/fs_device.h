@@ -53,19 +53,6 @@ struct Problem {
   const double *pulse_re, *pulse_im, *probe;
 };
 
-// Per-(s, e, j) geometry that does not depend on frequency.
-struct Geo {
-  double t;     // r_c / c
-  float amp;    // obliquity * directivity (centre frequency unless fulldir) / sqrt(r_c) / n_sub
-  float att;    // Np / Hz
-  float sint;   // sin(theta), used by the per-frequency directivity
-};
-
-__device__ __forceinline__ float sinc_rad(float a) {
-  // sin(a) / a with the removable singularity at 0 (numpy.sinc(a / pi)).
-  return (a == 0.0f) ? 1.0f : sinf(a) / a;
-}
-
 __device__ __forceinline__ bool out_of_field(const Problem& p, float xs, float zs) {
   bool out = zs < 0.0f;
   if (isfinite(p.radius)) {
@@ -75,68 +62,11 @@ __device__ __forceinline__ bool out_of_field(const Problem& p, float xs, float z
   return out;
 }
 
-__device__ __forceinline__ Geo geometry(const Problem& p, double xs, double zs, int e, int j) {
-  const int idx = e * p.NS + j;
-  const double dx = xs - p.sub_x[idx];
-  const double dz = zs - p.sub_z[idx];
-  const double r = sqrt(fma(dx, dx, dz * dz));
-  const double rcl = fmax(r, p.min_dist);
-  const float rinv = 1.0f / (float)(r + 1e-16);
-  const float sa = (float)(dx + 1e-16) * rinv;  // sin(asin(u)) with the reference's 1e-16 guard
-  const float ca = fabsf((float)dz) * rinv;     // cos(asin(u)) = |dz| / r
-  const float ce = (float)p.ecos[e], se = (float)p.esin[e];
-  const float st = sa * ce - ca * se;           // sin(asin(u) - theta_e)
-  const float ct = ca * ce + sa * se;           // cos(asin(u) - theta_e)
-  float obl;
-  if (p.baffle == FS_BAFFLE_SOFT) {
-    obl = ct;
-  } else if (p.baffle == FS_BAFFLE_RIGID) {
-    obl = 1.0f;
-  } else {
-    obl = ct / (ct + p.baffle_b);
-  }
-  if (!(ct > 0.0f)) obl = 1e-16f;               // |theta| >= pi/2
-  const float rcf = (float)rcl;
-  float amp = obl * rsqrtf(rcf) * p.inv_nsub;
-  if (!p.fulldir) amp *= sinc_rad(p.kappa * st);
-  Geo g;
-  g.t = rcl * p.inv_c;
-  g.amp = amp;
-  g.att = p.att_coef * rcf;
-  g.sint = st;
-  return g;
-}
-
-// exp(-att f) exp(i 2 pi frac(f t)) * amp, with the phase reduced in FP64.
-__device__ __forceinline__ cf green_at(const Geo& g, double f) {
-  const double nu = f * g.t;
-  const float ph = (float)(nu - floor(nu));
-  float s, c;
-  sincospif(2.0f * ph, &s, &c);
-  const float m = (g.att != 0.0f) ? g.amp * expf(-g.att * (float)f) : g.amp;
-  return {m * c, m * s};
-}
-
-// Per-step multiplier exp((-att + i 2 pi t) df).
-__device__ __forceinline__ cf green_step(const Geo& g, double df) {
-  const double nu = df * g.t;
-  const float ph = (float)(nu - floor(nu));
-  float s, c;
-  sincospif(2.0f * ph, &s, &c);
-  const float m = (g.att != 0.0f) ? expf(-g.att * (float)df) : 1.0f;
-  return {m * c, m * s};
-}
-
-// Per-frequency directivity sinc(k_f seg/2 sin(theta) / pi).
-__device__ __forceinline__ float dir_at(const Problem& p, const Geo& g, double f) {
-  return sinc_rad(p.kappa_per_hz * (float)f * g.sint);
-}
-
-// ---------------------------------------------------------------- fast variants
-// Same quantities as geometry() / green_at(), evaluated with the gfx950
-// hardware transcendentals: v_sin_f32 / v_cos_f32 take revolutions (max abs
-// error 1.3e-7 measured over [0, 1)), v_exp_f32 (rel 8e-8), v_rsq_f64 refined
-// by one Newton step (rel ~1e-15). The phase is still reduced in FP64.
+// ---------------------------------------------------------------- per-term geometry
+// Evaluated with the gfx950 hardware transcendentals: v_sin_f32 / v_cos_f32
+// take revolutions (max abs error 1.3e-7 measured over [0, 1)), v_exp_f32
+// (rel 8e-8), v_rsq_f64 refined by one Newton step (rel ~1e-15). The phase is
+// reduced in FP64.
 constexpr float kLog2e = 1.4426950408889634f;
 
 __device__ __forceinline__ float sinc_fast(float a) {
