@@ -283,6 +283,74 @@ def bench_adjoint(args) -> int:
     return 0
 
 
+def bench_pfield(args) -> int:
+    """RMS pressure field of one focused transmit of the 128-element linear array on a 512 x 512 grid
+    (x in [-2, 2] cm, z in [0.2, 5] cm); one step = fs_pfield over the grid, inputs in HBM."""
+    import torch
+
+    import fastsimus_amd as fs
+    from fastsimus_amd import _lib
+    from fastsimus_amd import kernels as K
+    from oracle import simus_oracle as O
+
+    dev = torch.device("cuda", 0)
+    params = fs.TransducerParams(freq_center=7.6e6, pitch=3.0e-4, width=2.7e-4, n_elements=128, bandwidth=0.77)
+    medium = fs.MediumParams(speed_of_sound=1540.0, attenuation=0.5)
+    gx, gz = np.meshgrid(np.linspace(-0.02, 0.02, 512), np.linspace(0.002, 0.05, 512), indexing="ij")
+    grid = np.stack([gx, gz], -1).astype(np.float32)
+    pos, _, _ = fs.element_positions(params.n_elements, params.pitch, params.radius)
+    delays = fs.focused(pos, np.array([0.0, 0.02]), speed_of_sound=1540.0)
+    plan = fs.pfield_precompute(grid, delays, params, medium)
+    prob = K.Problem(grid, None, delays, None, params, medium, n_sub=plan.n_sub, seg_length=plan.seg_length,
+                     freqs=plan.selected_freqs, pulse=plan.pulse_spectrum, probe=plan.probe_spectrum,
+                     correction=plan.correction_factor, full_frequency_directivity=False, device=dev)
+    lib = _lib.load()
+    ws = K._workspace(lib.fs_pfield_workspace_bytes(prob.dref), dev)
+    out = torch.empty(prob.x.numel(), dtype=torch.float32, device=dev)
+
+    def step():
+        _lib.check(lib.fs_pfield(prob.dref, out.data_ptr(), ws.data_ptr(), ws.numel(), K._stream(dev)), "fs_pfield")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    nf = int(plan.selected_freqs.shape[0])
+    npts = grid.shape[0] * grid.shape[1]
+    evals = npts * params.n_elements * nf
+    # FP64 oracle on a sample of grid points (same plan)
+    rng = np.random.default_rng(11)
+    idx = rng.choice(npts, 400, replace=False)
+    flat = grid.reshape(-1, 2).astype(np.float64)
+    plan_o = O.pfield_plan(flat[:, 0], flat[:, 1], delays, params, medium)
+    t0 = time.perf_counter()
+    want = O.pfield_oracle(flat[idx, 0], flat[idx, 1], delays, params, medium, plan_o)
+    t_cpu = time.perf_counter() - t0
+    err = rel_l2(out.cpu().numpy()[idx], want)
+    fl = evals * (8 * plan.n_sub + 8)
+    res = {
+        "metric": "G(point·element·freq)/s, pfield (RMS pressure field)", "value": evals / t_step / 1e9,
+        "unit": "G(point·element·freq)/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "pfield: 128-element 7.6 MHz linear array focused at 2 cm, 512x512 grid over "
+                               "[-2,2] x [0.2,5] cm, 0.5 dB/cm/MHz", "n_points": npts, "n_elem": params.n_elements,
+                   "n_freq": nf, "n_sub": plan.n_sub},
+        "roofline": {"bound": "valu", "achieved": fl / t_step / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                     "frac": fl / t_step / PEAK_FP32, "traffic": None, "kernel": "k_tx (pfield mode)"},
+        "rf_rel_l2": {"pfield_vs_fp64": err, "fp64_sample": "400 grid points"},
+        "cpu_baseline": {"value": 400 * params.n_elements * plan_o["freqs"].size / t_cpu / 1e9,
+                         "unit": "G(point·element·freq)/s", "cores": 1, "kind": "port",
+                         "sample": f"400 of {npts} grid points, FP64 NumPy, {t_cpu:.1f} s"},
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,9 +361,10 @@ def main() -> int:
     ap.add_argument("--plain-tx", type=int, default=3, help="transmits timed with the plain FP32 kernel")
     ap.add_argument("--cpu-sample", type=int, default=1000, help="scatterers in the CPU-baseline / FP64 check sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("planewave75", "phantoms64", "adjoint15"), default="planewave75",
+    ap.add_argument("--workload", choices=("planewave75", "phantoms64", "adjoint15", "pfield"), default="planewave75",
                     help="planewave75 (default, the headline metric), phantoms64 (uneven phantom batch, phased "
-                         "array, focused beams) or adjoint15 (forward + gradient of the differentiable form)")
+                         "array, focused beams), adjoint15 (forward + gradient of the differentiable form) or "
+                         "pfield (RMS pressure field on a grid)")
     ap.add_argument("--n-phantoms", type=int, default=64)
     ap.add_argument("--no-tune", action="store_true", help="skip the tile-shape autotuner (library defaults)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -327,6 +396,8 @@ def main() -> int:
         return bench_phantoms(args)
     if args.workload == "adjoint15":
         return bench_adjoint(args)
+    if args.workload == "pfield":
+        return bench_pfield(args)
     params, medium, x, z, rc, delays = workload(n_scat=args.n_scat, n_tx=args.n_tx)
     sc = np.stack([x, z], -1)
     plan = fs.simus_precompute(sc, rc, delays, params, medium)
