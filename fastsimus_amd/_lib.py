@@ -61,7 +61,7 @@ class Tuning(C.Structure):
     """Mirror of ``fs_tuning`` (zero fields mean default)."""
 
     _fields_ = [("msm_fbk", C.c_int32), ("msm_units", C.c_int32), ("msm_item_div", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("msm_table", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 _lib = None

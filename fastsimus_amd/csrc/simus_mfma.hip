@@ -28,8 +28,10 @@
 // Workgroup = ET waves (one thread per scatterer quad and element), one block
 // of FBK frequencies and one chunk of scatterers. Per tile of 16 scatterers:
 //   1. every thread evaluates G for 4 scatterers x 1 element at the FBK
-//      frequencies (FP64 distance and phase seed once per block, FP32
-//      recurrence inside it) and writes split-bf16 fragments to LDS;
+//      frequencies (FP64 phase seed once per block, FP32 recurrence inside
+//      it; the frequency-independent distance, amplitude and step come from
+//      a table built once per call, or are evaluated per block where no table
+//      is used) and writes split-bf16 fragments to LDS;
 //   2. every wave runs its share of the upper-triangular 16x16 tiles of K_f
 //      for its frequencies, with accumulators in registers.
 // The LDS holds two tiles: tile n is multiplied while tile n + 1 is generated,
@@ -53,6 +55,7 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx2 __attribute__((ext_vector_type(2)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 
 constexpr int MST = 16;  // scatterers per tile (32-deep contraction: re + im)
 
@@ -151,11 +154,14 @@ struct MsmShape {
 // Green's-function variants of k_msm: FAST = soft baffle, centre-frequency
 // directivity with |kappa sin(theta)| <= 2.5 (branch-free); GENERIC = any
 // baffle, centre-frequency directivity; FULLDIR = directivity at every
-// frequency. Distances below 1e-15 m are taken as 1e-15 m (both are clipped to
-// c / (2 fc) for the amplitude and phase), which only moves the angle of a
-// scatterer sitting on a sub-element centroid; GENERIC and FULLDIR handle an
-// exact zero distance as the reference does (angle pi/2).
-enum MsmMode { MSM_FAST = 0, MSM_GENERIC = 1, MSM_FULLDIR = 2 };
+// frequency; TABLE = the frequency-independent part (clipped distance and
+// log2 of the amplitude) read from a table that k_geo_table fills once per
+// call, for any baffle with a non-negative amplitude and centre-frequency
+// directivity. Distances below 1e-15 m are taken as 1e-15 m (both are clipped
+// to c / (2 fc) for the amplitude and phase), which only moves the angle of a
+// scatterer sitting on a sub-element centroid; GENERIC, FULLDIR and TABLE
+// handle an exact zero distance as the reference does (angle pi/2).
+enum MsmMode { MSM_FAST = 0, MSM_GENERIC = 1, MSM_FULLDIR = 2, MSM_TABLE = 3 };
 
 // sin(a)/a for |a| <= 2.5: Taylor series to a^14 (truncation < 5e-9 on that range).
 __device__ __forceinline__ float sinc_poly_2p5(float a) {
@@ -240,6 +246,58 @@ __global__ void k_scat_prep(Problem p, float4* __restrict__ out) {
   const bool live = r != 0.f && !out_of_field(p, x, z);
   const float w = live ? copysignf(sqrtf(fabsf(r)) * p.inv_nsub, r) : 0.f;
   out[s] = make_float4(x, z, w, 0.f);
+}
+
+// ---------------------------------------------------------------- geometry table
+// Frequency-independent part of every (scatterer, element, sub-element) term,
+// evaluated once per call in FP64 (reference math: _pfield_math.py:71-247):
+//   rt = r_c = max(r, c / (2 fc))                         (m, FP64)
+//   la = log2(obliquity(theta) directivity(theta) |w| / sqrt(r_c))
+//   qt = exp((-att + i 2 pi / c) r_c df)                  (the per-frequency step)
+// with w = sqrt|rc| / n_sub from k_scat_prep (the sign of rc is applied in
+// k_msm), directivity at the centre frequency, theta = asin(dx / r) - theta_e
+// (pi/2 at r = 0). la = -inf for dead scatterers and padded elements, whose
+// rt is left finite (c / (2 fc)) so that their terms are exactly zero.
+// Layout [s][e][j] with EP (padded element count) elements per scatterer, so
+// that a lane of k_msm reads its NSUB terms of one scatterer as one vector load.
+// Requires a non-negative amplitude: sinc(kappa sin(theta)) >= 0 (kappa <= pi)
+// and an obliquity >= 0 (soft, rigid, or impedance b >= 0).
+template <int NSUB>
+__global__ void k_geo_table(Problem p, const float4* __restrict__ scat, int EP, double* __restrict__ rt,
+                            float* __restrict__ la, float2* __restrict__ qt) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)p.S * EP) return;
+  const int s = (int)(idx / EP);
+  const int e = (int)(idx - (long long)s * EP);
+  const float4 sd = scat[s];
+  const double w = fabs((double)sd.z);
+#pragma unroll
+  for (int j = 0; j < NSUB; ++j) {
+    double rcl = p.min_dist, l = -INFINITY;
+    if (e < p.E && w > 0.0) {
+      const double dx = (double)sd.x - p.sub_x[e * NSUB + j];
+      const double dz = (double)sd.y - p.sub_z[e * NSUB + j];
+      const double r = sqrt(fma(dx, dx, dz * dz));
+      rcl = fmax(r, p.min_dist);
+      const double sa = r > 0.0 ? dx / r : 1.0;
+      const double ca = r > 0.0 ? fabs(dz) / r : 0.0;
+      const double ce = p.ecos[e], se = p.esin[e];
+      const double st = sa * ce - ca * se;
+      const double ct = ca * ce + sa * se;
+      double obl = (p.baffle == FS_BAFFLE_SOFT) ? ct : (p.baffle == FS_BAFFLE_RIGID) ? 1.0 : ct / (ct + p.baffle_b);
+      if (!(ct > 0.0)) obl = 1e-16;
+      const double a = (double)p.kappa * st;
+      const double dir = fabs(a) < 1e-4 ? 1.0 - a * a / 6.0 : sin(a) / a;
+      const double amp = obl * dir * w / sqrt(rcl);
+      l = amp > 0.0 ? log2(amp) : -INFINITY;
+    }
+    double qs, qc;
+    sincospi(2.0 * (p.df * p.inv_c * rcl), &qs, &qc);
+    const double qm = exp(-(double)p.att_coef * p.df * rcl);
+    rt[idx * NSUB + j] = rcl;
+    la[idx * NSUB + j] = (float)l;
+    qt[idx * NSUB + j] = make_float2((float)(qm * qc), (float)(qm * qs));
+  }
 }
 
 // ---------------------------------------------------------------- G fragments
@@ -338,6 +396,112 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<N
   }
 }
 
+// The NSUB table entries of one (scatterer, element): one vector load each.
+template <int NSUB>
+struct TableTerm {
+  double r[NSUB];
+  float l[NSUB];
+  float2 q[NSUB];
+};
+// row = s * EP is wave-uniform: the scalar part of the address; e the lane's offset.
+template <int NSUB>
+__device__ __forceinline__ TableTerm<NSUB> load_term(const double* __restrict__ rt, const float* __restrict__ la,
+                                                     const float2* __restrict__ qt, size_t row, int e) {
+  TableTerm<NSUB> t;
+  if constexpr (NSUB == 2) {
+    // byte offsets as unsigned 32-bit: scalar base + vector offset addressing
+    const unsigned o8 = (unsigned)e * 8u, o16 = (unsigned)e * 16u;
+    const double2 r = *(const double2*)((const char*)(rt + row * 2) + o16);
+    const float2 l = *(const float2*)((const char*)(la + row * 2) + o8);
+    const float4 q = *(const float4*)((const char*)(qt + row * 2) + o16);
+    t.r[0] = r.x;
+    t.r[1] = r.y;
+    t.l[0] = l.x;
+    t.l[1] = l.y;
+    t.q[0] = make_float2(q.x, q.y);
+    t.q[1] = make_float2(q.z, q.w);
+  } else {
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+      t.r[j] = (rt + row * NSUB)[e * NSUB + j];
+      t.l[j] = (la + row * NSUB)[e * NSUB + j];
+      t.q[j] = (qt + row * NSUB)[e * NSUB + j];
+    }
+  }
+  return t;
+}
+
+// One thread's table entries for one tile (its 4 scatterers).
+template <int NSUB>
+struct TableTile {
+  TableTerm<NSUB> t[4];
+  int wbits[4];   // bits of w = +-sqrt|rc| / n_sub: the sign of rc (wave-uniform)
+  bool valid[4];  // scatterer inside the unit (wave-uniform)
+};
+
+template <int NSUB, int ET>
+__device__ __forceinline__ void fetch_table(const float4* __restrict__ scat, const double* __restrict__ rt,
+                                            const float* __restrict__ la, const float2* __restrict__ qt, int s0,
+                                            int s_end, TableTile<NSUB>& tt) {
+  constexpr int EP = ET * 16;
+  const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / EP));  // wave-uniform
+  const int e = threadIdx.x % EP;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int s = s0 + 4 * g + i;
+    tt.valid[i] = s < s_end;
+    const int sc = tt.valid[i] ? s : s_end - 1;  // in-bounds; masked in gen_fragments_table
+    tt.wbits[i] = __float_as_int(scat[sc].z);
+    tt.t[i] = load_term<NSUB>(rt, la, qt, (size_t)sc * EP, e);
+  }
+}
+
+// Fragments from the geometry table: per (scatterer, sub-element) only the
+// block's seed is evaluated here: the phase frac(f_b r_c / c + quarter) in
+// FP64 and the magnitude 2^(la - att f_b r_c log2 e); the block's FBK
+// frequencies follow by the FP32 recurrence with the tabulated step q, as in
+// gen_fragments.
+template <int NSUB, int FBK, int ET>
+__device__ __forceinline__ void gen_fragments_table(const BlockScalars& bs, const TableTile<NSUB>& tt, u32x4* frag) {
+  using L = FragLds<ET, FBK>;
+  constexpr int EP = ET * 16;
+  const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / EP));  // wave-uniform
+  const int e = threadIdx.x % EP;
+  float re[FBK][4], im[FBK][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double quarter = tt.wbits[i] < 0 ? 0.25 : 0.0;
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+      const double r = tt.t[i].r[j];
+      const float lj = tt.valid[i] ? tt.t[i].l[j] : -INFINITY;
+      const float rf = (float)r;
+      const float ph0 = (float)__builtin_amdgcn_fract(fma(bs.nu0, r, quarter));  // v_fract_f64: x - floor(x)
+      const float m0 = __builtin_amdgcn_exp2f(fmaf(-bs.a0, rf, lj));
+      cf gv = {m0 * __builtin_amdgcn_cosf(ph0), m0 * __builtin_amdgcn_sinf(ph0)};
+      const cf q = {tt.t[i].q[j].x, tt.t[i].q[j].y};
+#pragma unroll
+      for (int k = 0; k < FBK; ++k) {
+        re[k][i] = j == 0 ? gv.x : re[k][i] + gv.x;
+        im[k][i] = j == 0 ? gv.y : im[k][i] + gv.y;
+        if (k + 1 < FBK) gv = cmul(gv, q);
+      }
+    }
+  }
+  const int tile = e >> 4;
+  const int lane = (e & 15) + 16 * g;
+#pragma unroll
+  for (int k = 0; k < FBK; ++k) {
+    unsigned h0, h1, h2, h3, l0, l1, l2, l3;
+    split_pair(re[k][0], re[k][1], h0, l0);
+    split_pair(re[k][2], re[k][3], h1, l1);
+    split_pair(im[k][0], im[k][1], h2, l2);
+    split_pair(im[k][2], im[k][3], h3, l3);
+    frag[L::idx(k, 0, tile, lane)] = (u32x4){h0, h1, h2, h3};
+    frag[L::idx(k, 1, tile, lane)] = (u32x4){l0, l1, l2, l3};
+  }
+}
+
 // ---------------------------------------------------------------- MFMA tiles of one wave
 // Accumulators of one wave: its job list at the KPW frequencies k0 .. k0 + KPW - 1 of the block.
 template <int ET, int FBK, int KPW, int W>
@@ -371,8 +535,10 @@ struct WaveTiles {
         const u32x4 l = frag[L::idx(k, 1, r, lane)];
         arh[r] = h ^ nim;
         arl[r] = l ^ nim;
-        aih[r] = (u32x4){h.z, h.w, h.x, h.y};
-        ail[r] = (u32x4){l.z, l.w, l.x, l.y};
+        // halves swapped as 64-bit moves
+        const u64x2 h64 = __builtin_bit_cast(u64x2, h), l64 = __builtin_bit_cast(u64x2, l);
+        aih[r] = __builtin_bit_cast(u32x4, (u64x2){h64.y, h64.x});
+        ail[r] = __builtin_bit_cast(u32x4, (u64x2){l64.y, l64.x});
       }
 #pragma unroll
       for (int c = 0; c < ET; ++c) {
@@ -421,8 +587,10 @@ struct WaveTiles {
 // iteration (one multiplies while the other generates) so that matrix-core and
 // vector work overlap; one barrier per tile separates the buffers' roles.
 template <int NSUB, int ET, int MODE, bool SHALLOW, int W>
-__device__ __forceinline__ void msm_wave(const Problem& p, const float4* __restrict__ scat, int sb, int se, int fb,
-                                         int k0, bool mfma_first, char* lds, cf* kpart) {
+__device__ __forceinline__ void msm_wave(const Problem& p, const float4* __restrict__ scat,
+                                         const double* __restrict__ rt, const float* __restrict__ la,
+                                         const float2* __restrict__ qt, int sb, int se, int fb, int k0,
+                                         bool mfma_first, char* lds, cf* kpart) {
   using SH = MsmShape<ET, SHALLOW>;
   constexpr int FBK = SH::FBK;
   using L = FragLds<ET, FBK>;
@@ -432,22 +600,45 @@ __device__ __forceinline__ void msm_wave(const Problem& p, const float4* __restr
   WaveTiles<ET, FBK, SH::KPW, W> t;
   t.zero();
   int buf = 0;
-  if (sb < se) gen_fragments<NSUB, FBK, MODE, ET>(p, ed, bs, scat, sb, se, frag);
-  __syncthreads();
-  for (int s0 = sb; s0 < se; s0 += MST) {
-    const bool more = s0 + MST < se;
-    const int nb = buf ^ 1;
-    if (mfma_first) t.step(frag + buf * L::per_buf, k0);
-    if (more) {
-      // The generate-first wave of a SIMD gets the vector units first, so that its
-      // multiply phase starts while the other wave still has vector work to overlap.
-      if (!mfma_first) __builtin_amdgcn_s_setprio(1);
-      gen_fragments<NSUB, FBK, MODE, ET>(p, ed, bs, scat, s0 + MST, se, frag + nb * L::per_buf);
-      __builtin_amdgcn_s_setprio(0);
+  if constexpr (MODE == MSM_TABLE) {
+    TableTile<NSUB> nxt;
+    if (sb < se) {
+      fetch_table<NSUB, ET>(scat, rt, la, qt, sb, se, nxt);
+      gen_fragments_table<NSUB, FBK, ET>(bs, nxt, frag);
     }
-    if (!mfma_first) t.step(frag + buf * L::per_buf, k0);
     __syncthreads();
-    buf = nb;
+    for (int s0 = sb; s0 < se; s0 += MST) {
+      const bool more = s0 + MST < se;
+      const int nb = buf ^ 1;
+      if (mfma_first) t.step(frag + buf * L::per_buf, k0);
+      if (more) {
+        if (!mfma_first) __builtin_amdgcn_s_setprio(1);
+        fetch_table<NSUB, ET>(scat, rt, la, qt, s0 + MST, se, nxt);
+        gen_fragments_table<NSUB, FBK, ET>(bs, nxt, frag + nb * L::per_buf);
+        __builtin_amdgcn_s_setprio(0);
+      }
+      if (!mfma_first) t.step(frag + buf * L::per_buf, k0);
+      __syncthreads();
+      buf = nb;
+    }
+  } else {
+    if (sb < se) gen_fragments<NSUB, FBK, MODE, ET>(p, ed, bs, scat, sb, se, frag);
+    __syncthreads();
+    for (int s0 = sb; s0 < se; s0 += MST) {
+      const bool more = s0 + MST < se;
+      const int nb = buf ^ 1;
+      if (mfma_first) t.step(frag + buf * L::per_buf, k0);
+      if (more) {
+        // The generate-first wave of a SIMD gets the vector units first, so that its
+        // multiply phase starts while the other wave still has vector work to overlap.
+        if (!mfma_first) __builtin_amdgcn_s_setprio(1);
+        gen_fragments<NSUB, FBK, MODE, ET>(p, ed, bs, scat, s0 + MST, se, frag + nb * L::per_buf);
+        __builtin_amdgcn_s_setprio(0);
+      }
+      if (!mfma_first) t.step(frag + buf * L::per_buf, k0);
+      __syncthreads();
+      buf = nb;
+    }
   }
   t.store(kpart, ET * 16, fb * FBK + k0);
 }
@@ -478,7 +669,9 @@ struct MsmPhantoms {
 // before its tile loop and one per tile.
 template <int NSUB, int ET, int MODE, bool SHALLOW, int W>
 __device__ __forceinline__ void msm_persistent(const Problem& p, const float4* __restrict__ scat,
-                                               const MsmUnits& units, int* __restrict__ queue, int nfpad,
+                                               const double* __restrict__ rt, const float* __restrict__ la,
+                                               const float2* __restrict__ qt, const MsmUnits& units,
+                                               int* __restrict__ queue, int nfpad,
                                                cf* __restrict__ kpart, int k0, bool mfma_first, char* lds,
                                                int* next_item) {
   const int n_items = units.n * units.nfb;
@@ -492,7 +685,7 @@ __device__ __forceinline__ void msm_persistent(const Problem& p, const float4* _
     const int fb = item - ui * units.nfb;
     const MsmUnit u = units.u[ui];
     cf* kp = kpart + (size_t)u.slot * nfpad * (ET * 16) * (ET * 16);
-    msm_wave<NSUB, ET, MODE, SHALLOW, W>(p, scat, u.s_begin, u.s_end, fb, k0, mfma_first, lds, kp);
+    msm_wave<NSUB, ET, MODE, SHALLOW, W>(p, scat, rt, la, qt, u.s_begin, u.s_end, fb, k0, mfma_first, lds, kp);
   }
 }
 
@@ -501,8 +694,11 @@ __device__ __forceinline__ void msm_persistent(const Problem& p, const float4* _
 // and w + 4 (and w, w + 1) take opposite orders of multiply and generate.
 template <int NSUB, int ET, int MODE, bool SHALLOW>
 __global__ __launch_bounds__(64 * ET, ET == 8 ? 1 : 2) void k_msm(Problem p, const float4* __restrict__ scat,
-                                                                 const MsmUnits units, int* __restrict__ queue,
-                                                                 int nfpad, cf* __restrict__ kpart) {
+                                                                 const double* __restrict__ rt,
+                                                                 const float* __restrict__ la,
+                                                                 const float2* __restrict__ qt, const MsmUnits units,
+                                                                 int* __restrict__ queue, int nfpad,
+                                                                 cf* __restrict__ kpart) {
   using SH = MsmShape<ET, SHALLOW>;
   extern __shared__ __attribute__((aligned(16))) char lds_raw[];
   __shared__ int next_item;
@@ -511,16 +707,16 @@ __global__ __launch_bounds__(64 * ET, ET == 8 ? 1 : 2) void k_msm(Problem p, con
   const bool mfma_first = ((w ^ (w >> 2)) & 1) == 0;
   if constexpr (SH::WPP == 4) {
     switch (w % 4) {
-      case 0: msm_persistent<NSUB, ET, MODE, SHALLOW, 0>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
-      case 1: msm_persistent<NSUB, ET, MODE, SHALLOW, 1>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
-      case 2: msm_persistent<NSUB, ET, MODE, SHALLOW, 2>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
-      default: msm_persistent<NSUB, ET, MODE, SHALLOW, 3>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
+      case 0: msm_persistent<NSUB, ET, MODE, SHALLOW, 0>(p, scat, rt, la, qt, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
+      case 1: msm_persistent<NSUB, ET, MODE, SHALLOW, 1>(p, scat, rt, la, qt, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
+      case 2: msm_persistent<NSUB, ET, MODE, SHALLOW, 2>(p, scat, rt, la, qt, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
+      default: msm_persistent<NSUB, ET, MODE, SHALLOW, 3>(p, scat, rt, la, qt, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
     }
   } else {
     if (w % 2 == 0)
-      msm_persistent<NSUB, ET, MODE, SHALLOW, 0>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item);
+      msm_persistent<NSUB, ET, MODE, SHALLOW, 0>(p, scat, rt, la, qt, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item);
     else
-      msm_persistent<NSUB, ET, MODE, SHALLOW, 1>(p, scat, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item);
+      msm_persistent<NSUB, ET, MODE, SHALLOW, 1>(p, scat, rt, la, qt, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item);
   }
 }
 
@@ -608,8 +804,8 @@ __global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, const M
 // ---------------------------------------------------------------- host side
 struct MsmPlan {
   int ET, FBK, nfb, nfpad, threads, grid, n_items;
-  bool shallow;
-  size_t lds, kpart_bytes, scat_offset, queue_offset, ws_bytes;
+  bool shallow, table;
+  size_t lds, kpart_bytes, scat_offset, rt_offset, la_offset, qt_offset, queue_offset, ws_bytes;
   MsmUnits units;
   MsmPhantoms phantoms;
 };
@@ -643,6 +839,13 @@ static bool msm_fast_mode(const Problem& p) { return !p.fulldir && p.baffle == F
 
 static bool shallow_required(const Problem& p) { return p.NS > 2 || p.fulldir; }
 
+// The geometry table holds log2 of the amplitude, so it needs amplitudes >= 0:
+// centre-frequency directivity with kappa <= pi and a non-negative obliquity.
+static bool table_possible(const Problem& p) {
+  return !p.fulldir && p.kappa <= kPi && !(p.baffle == FS_BAFFLE_IMPEDANCE && !(p.baffle_b >= 0.f));
+}
+constexpr size_t MSM_TABLE_AUTO_MAX = (size_t)32 << 30;  // bytes of table used without being asked
+
 static std::array<long long, 8> tuning_key(const Problem& p, int P) {
   long long tiles = std::max(1, (p.S + MST - 1) / MST), lg = 0;
   while ((1LL << (lg + 1)) <= tiles) ++lg;
@@ -668,7 +871,7 @@ int msm_tuning_set(const Problem& p, int P, const fs_tuning* t) {
     return FS_OK;
   }
   if ((t->msm_fbk != 0 && t->msm_fbk != 2 && t->msm_fbk != 4) || t->msm_units < 0 || t->msm_units > MSM_MAX_UNITS ||
-      t->msm_item_div < 0 || t->msm_item_div > 1024)
+      t->msm_item_div < 0 || t->msm_item_div > 1024 || t->msm_table < 0 || t->msm_table > 2)
     return FS_ERR_INVALID;
   tuning_table[tuning_key(p, P)] = *t;
   return FS_OK;
@@ -750,7 +953,13 @@ static int msm_plan_et(const Problem& p, int P, const int64_t* off, const fs_tun
   const size_t ep = (size_t)ET * 16;
   m.kpart_bytes = (size_t)m.units.n * m.nfpad * ep * ep * sizeof(cf);
   m.scat_offset = (m.kpart_bytes + 255) / 256 * 256;
-  m.queue_offset = m.scat_offset + ((size_t)off[P] * sizeof(float4) + 255) / 256 * 256;
+  m.rt_offset = m.scat_offset + ((size_t)off[P] * sizeof(float4) + 255) / 256 * 256;
+  const size_t terms = (size_t)off[P] * ep * p.NS;
+  const size_t table_bytes = terms * (sizeof(double) + sizeof(float) + sizeof(float2));
+  m.table = table_possible(p) && tune.msm_table != 2 && (tune.msm_table == 1 || table_bytes <= MSM_TABLE_AUTO_MAX);
+  m.la_offset = m.rt_offset + (m.table ? (terms * sizeof(double) + 255) / 256 * 256 : 0);
+  m.qt_offset = m.la_offset + (m.table ? (terms * sizeof(float) + 255) / 256 * 256 : 0);
+  m.queue_offset = m.qt_offset + (m.table ? (terms * sizeof(float2) + 255) / 256 * 256 : 0);
   m.ws_bytes = m.queue_offset + 256;
   m.lds = FragLds<ET, SH::FBK>::bytes;
   return FS_OK;
@@ -781,8 +990,8 @@ size_t mfma_workspace(const Problem& p) {
 }
 
 template <int NSUB, int ET, int MODE, bool SHALLOW>
-static int launch_msm(const Problem& p, const MsmPlan& m, const float4* scat, int* queue, cf* kpart,
-                      hipStream_t st) {
+static int launch_msm(const Problem& p, const MsmPlan& m, const float4* scat, const double* rt, const float* la,
+                      const float2* qt, int* queue, cf* kpart, hipStream_t st) {
   auto kern = k_msm<NSUB, ET, MODE, SHALLOW>;
   static bool lds_set[MAX_DEVICES] = {false};  // dynamic LDS above 64 KiB: set once per device
   const int dev = current_device();
@@ -793,33 +1002,45 @@ static int launch_msm(const Problem& p, const MsmPlan& m, const float4* scat, in
   }
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, queue, 0);
   const int slot = timing_start(st);
-  hipLaunchKernelGGL(kern, dim3(m.grid), dim3(m.threads), m.lds, st, p, scat, m.units, queue, m.nfpad, kpart);
+  hipLaunchKernelGGL(kern, dim3(m.grid), dim3(m.threads), m.lds, st, p, scat, rt, la, qt, m.units, queue, m.nfpad,
+                     kpart);
   timing_stop(slot, st);
   return FS_OK;
 }
 
 template <int NSUB, int ET>
-static int launch_msm_fd(const Problem& p, const MsmPlan& m, const float4* scat, int* queue, cf* kpart,
-                         hipStream_t st) {
+static int launch_msm_fd(const Problem& p, const MsmPlan& m, const float4* scat, const double* rt, const float* la,
+                         const float2* qt, int* queue, cf* kpart, hipStream_t st) {
+  if (m.table) {
+    if constexpr (NSUB > 2) return launch_msm<NSUB, ET, MSM_TABLE, true>(p, m, scat, rt, la, qt, queue, kpart, st);
+    else
+      return m.shallow ? launch_msm<NSUB, ET, MSM_TABLE, true>(p, m, scat, rt, la, qt, queue, kpart, st)
+                       : launch_msm<NSUB, ET, MSM_TABLE, false>(p, m, scat, rt, la, qt, queue, kpart, st);
+  }
   if constexpr (NSUB > 2) {
-    if (p.fulldir) return launch_msm<NSUB, ET, MSM_FULLDIR, true>(p, m, scat, queue, kpart, st);
-    if (msm_fast_mode(p)) return launch_msm<NSUB, ET, MSM_FAST, true>(p, m, scat, queue, kpart, st);
-    return launch_msm<NSUB, ET, MSM_GENERIC, true>(p, m, scat, queue, kpart, st);
+    if (p.fulldir) return launch_msm<NSUB, ET, MSM_FULLDIR, true>(p, m, scat, rt, la, qt, queue, kpart, st);
+    if (msm_fast_mode(p)) return launch_msm<NSUB, ET, MSM_FAST, true>(p, m, scat, rt, la, qt, queue, kpart, st);
+    return launch_msm<NSUB, ET, MSM_GENERIC, true>(p, m, scat, rt, la, qt, queue, kpart, st);
   } else {
-    if (p.fulldir) return launch_msm<NSUB, ET, MSM_FULLDIR, true>(p, m, scat, queue, kpart, st);
+    if (p.fulldir) return launch_msm<NSUB, ET, MSM_FULLDIR, true>(p, m, scat, rt, la, qt, queue, kpart, st);
     if (msm_fast_mode(p))
-      return m.shallow ? launch_msm<NSUB, ET, MSM_FAST, true>(p, m, scat, queue, kpart, st)
-                       : launch_msm<NSUB, ET, MSM_FAST, false>(p, m, scat, queue, kpart, st);
-    return m.shallow ? launch_msm<NSUB, ET, MSM_GENERIC, true>(p, m, scat, queue, kpart, st)
-                     : launch_msm<NSUB, ET, MSM_GENERIC, false>(p, m, scat, queue, kpart, st);
+      return m.shallow ? launch_msm<NSUB, ET, MSM_FAST, true>(p, m, scat, rt, la, qt, queue, kpart, st)
+                       : launch_msm<NSUB, ET, MSM_FAST, false>(p, m, scat, rt, la, qt, queue, kpart, st);
+    return m.shallow ? launch_msm<NSUB, ET, MSM_GENERIC, true>(p, m, scat, rt, la, qt, queue, kpart, st)
+                     : launch_msm<NSUB, ET, MSM_GENERIC, false>(p, m, scat, rt, la, qt, queue, kpart, st);
   }
 }
 
 template <int NSUB>
-static int launch_msm_et(const Problem& p, const MsmPlan& m, const float4* scat, int* queue, cf* kpart,
-                         hipStream_t st) {
-  return m.ET == 4 ? launch_msm_fd<NSUB, 4>(p, m, scat, queue, kpart, st)
-                   : launch_msm_fd<NSUB, 8>(p, m, scat, queue, kpart, st);
+static int launch_msm_et(const Problem& p, const MsmPlan& m, const float4* scat, double* rt, float* la, float2* qt,
+                         int* queue, cf* kpart, hipStream_t st) {
+  if (m.table && p.S > 0) {
+    const long long n = (long long)p.S * m.ET * 16;
+    hipLaunchKernelGGL(k_geo_table<NSUB>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, scat, m.ET * 16, rt,
+                       la, qt);
+  }
+  return m.ET == 4 ? launch_msm_fd<NSUB, 4>(p, m, scat, rt, la, qt, queue, kpart, st)
+                   : launch_msm_fd<NSUB, 8>(p, m, scat, rt, la, qt, queue, kpart, st);
 }
 
 // Spectra of P phantoms sharing one probe, transmit set and frequency plan:
@@ -835,12 +1056,15 @@ int mfma_spectrum_batch(const Problem& p, int P, const int64_t* off, cf* spect, 
   cf* kpart = (cf*)ws;
   float4* scat = (float4*)(ws + m.scat_offset);
   int* queue = (int*)(ws + m.queue_offset);
+  double* rt = m.table ? (double*)(ws + m.rt_offset) : nullptr;
+  float* la = m.table ? (float*)(ws + m.la_offset) : nullptr;
+  float2* qt = m.table ? (float2*)(ws + m.qt_offset) : nullptr;
   if (p.S > 0) hipLaunchKernelGGL(k_scat_prep, dim3((p.S + 255) / 256), dim3(256), 0, st, p, scat);
   switch (p.NS) {
-    case 1: rc = launch_msm_et<1>(p, m, scat, queue, kpart, st); break;
-    case 2: rc = launch_msm_et<2>(p, m, scat, queue, kpart, st); break;
-    case 3: rc = launch_msm_et<3>(p, m, scat, queue, kpart, st); break;
-    case 4: rc = launch_msm_et<4>(p, m, scat, queue, kpart, st); break;
+    case 1: rc = launch_msm_et<1>(p, m, scat, rt, la, qt, queue, kpart, st); break;
+    case 2: rc = launch_msm_et<2>(p, m, scat, rt, la, qt, queue, kpart, st); break;
+    case 3: rc = launch_msm_et<3>(p, m, scat, rt, la, qt, queue, kpart, st); break;
+    case 4: rc = launch_msm_et<4>(p, m, scat, rt, la, qt, queue, kpart, st); break;
     default: return FS_ERR_UNSUPPORTED;
   }
   if (rc != FS_OK) return rc;

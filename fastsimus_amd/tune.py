@@ -15,6 +15,10 @@ signature and stores the fastest in the library's tuning cache
 * ``msm_item_div`` (batches): the largest work item as a fraction of a queue
   slot's share.
 
+``msm_table`` (not searched: the timing covers ``k_msm`` only) selects
+whether the frequency-independent geometry comes from a per-call table
+(0: when possible and at most 32 GiB, 1: when possible, 2: never).
+
 The choice changes only the order of floating-point summations over
 scatterers (results agree to FP32 rounding). The reference's CUDA tile
 constants are fixed by hand (``kernels/cuda_simus.py:39-45``).
@@ -35,11 +39,12 @@ def get(prob: K.Problem, n_phantoms: int = 1) -> dict:
     """Tuning entry the library would use for this signature (zeros = defaults)."""
     t = _lib.Tuning()
     _lib.check(_lib.load().fs_tuning_get(prob.dref, n_phantoms, C.byref(t)), "fs_tuning_get")
-    return {"msm_fbk": t.msm_fbk, "msm_units": t.msm_units, "msm_item_div": t.msm_item_div}
+    return {"msm_fbk": t.msm_fbk, "msm_units": t.msm_units, "msm_item_div": t.msm_item_div,
+            "msm_table": t.msm_table}
 
 
 def set_tuning(prob: K.Problem, n_phantoms: int = 1, **fields) -> None:
-    """Store an entry (``msm_fbk``, ``msm_units``, ``msm_item_div``); no fields forgets the entry."""
+    """Store an entry (``msm_fbk``, ``msm_units``, ``msm_item_div``, ``msm_table``); no fields forgets the entry."""
     lib = _lib.load()
     if not fields:
         _lib.check(lib.fs_tuning_set(prob.dref, n_phantoms, None), "fs_tuning_set")

@@ -127,7 +127,10 @@ typedef struct fs_tuning {
   int32_t msm_fbk;       /* frequencies per block: 0 (default 4), 2 or 4; 2 is forced where registers need it */
   int32_t msm_units;     /* single phantom: scatterer chunks (0 = chosen from the CU count) */
   int32_t msm_item_div;  /* batches: largest work item <= 1/msm_item_div of a queue slot's share (0 = 8) */
-  int32_t reserved[5];
+  int32_t msm_table;     /* geometry table (per scatterer, element, sub-element: FP64 distance, log2 amplitude,
+                            frequency step; 20 B each, in the workspace): 0 = when possible and <= 32 GiB,
+                            1 = when possible, 2 = never (geometry evaluated per frequency block) */
+  int32_t reserved[4];
 } fs_tuning;
 /* t == NULL forgets the entry. n_phantoms >= 1 (1 for fs_simus_spectrum). */
 int fs_tuning_set(const fs_simus_desc* d, int n_phantoms, const fs_tuning* t);

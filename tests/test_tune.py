@@ -42,9 +42,9 @@ def test_tuning_set_get_and_validation(cuda):
     p, m, sc, rc, D, plan = _setup()
     prob = _prob(sc, rc, D, plan, p, m, cuda)
     tune.set_tuning(prob)  # forget
-    assert tune.get(prob) == {"msm_fbk": 0, "msm_units": 0, "msm_item_div": 0}
+    assert tune.get(prob) == {"msm_fbk": 0, "msm_units": 0, "msm_item_div": 0, "msm_table": 0}
     tune.set_tuning(prob, msm_fbk=2, msm_units=3)
-    assert tune.get(prob) == {"msm_fbk": 2, "msm_units": 3, "msm_item_div": 0}
+    assert tune.get(prob) == {"msm_fbk": 2, "msm_units": 3, "msm_item_div": 0, "msm_table": 0}
     assert tune.get(prob, 2)["msm_fbk"] == 0  # other signature untouched
     with pytest.raises(ValueError):
         tune.set_tuning(prob, msm_fbk=3)
@@ -67,6 +67,13 @@ def test_every_shape_matches_default_and_oracle(cuda):
         tune.set_tuning(prob, **shape)
         got = fs.simus_compute(sc, rc, D, plan, p, m, strategy="hip_mfma").rf.cpu().numpy()
         assert rel_l2(got, ref) < 1e-5, shape
+    # geometry per frequency block instead of the table: other FP32 rounding of the amplitudes
+    for shape in ({"msm_table": 2}, {"msm_table": 2, "msm_fbk": 2}, {"msm_table": 1, "msm_fbk": 2}):
+        print(shape, flush=True)
+        tune.set_tuning(prob, **shape)
+        got = fs.simus_compute(sc, rc, D, plan, p, m, strategy="hip_mfma").rf.cpu().numpy()
+        assert rel_l2(got, ref) < 3e-5, shape
+        assert rel_l2(got, want) < 1e-3, shape
     tune.set_tuning(prob)
 
 
