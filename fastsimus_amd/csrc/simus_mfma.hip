@@ -55,7 +55,8 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx2 __attribute__((ext_vector_type(2)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) u32x2 lds_u32x2;  // an LDS (local data share) location
 
 constexpr int MST = 16;  // scatterers per tile (32-deep contraction: re + im)
 
@@ -403,29 +404,40 @@ struct TableTerm {
   float l[NSUB];
   float2 q[NSUB];
 };
-// row = s * EP is wave-uniform: the scalar part of the address; e the lane's offset.
-template <int NSUB>
+// Buffer resource over one table row (wave-uniform base, the row's size in
+// bytes as its record count: lanes reading past it get zeros, not a fault).
+constexpr int kRawBufferWord3 = 0x00020000;  // raw (untyped) buffer, 32-bit data format
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, kRawBufferWord3);
+}
+
+// The NSUB entries of table row `row` = s * EP (wave-uniform) for element e,
+// by buffer loads: scalar row address plus the lane's 32-bit byte offset.
+template <int NSUB, int EP>
 __device__ __forceinline__ TableTerm<NSUB> load_term(const double* __restrict__ rt, const float* __restrict__ la,
                                                      const float2* __restrict__ qt, size_t row, int e) {
   TableTerm<NSUB> t;
+  const __amdgpu_buffer_rsrc_t rr = row_rsrc(rt + row * NSUB, EP * NSUB * 8);
+  const __amdgpu_buffer_rsrc_t rl = row_rsrc(la + row * NSUB, EP * NSUB * 4);
+  const __amdgpu_buffer_rsrc_t rq = row_rsrc(qt + row * NSUB, EP * NSUB * 8);
   if constexpr (NSUB == 2) {
-    // byte offsets as unsigned 32-bit: scalar base + vector offset addressing
-    const unsigned o8 = (unsigned)e * 8u, o16 = (unsigned)e * 16u;
-    const double2 r = *(const double2*)((const char*)(rt + row * 2) + o16);
-    const float2 l = *(const float2*)((const char*)(la + row * 2) + o8);
-    const float4 q = *(const float4*)((const char*)(qt + row * 2) + o16);
-    t.r[0] = r.x;
-    t.r[1] = r.y;
-    t.l[0] = l.x;
-    t.l[1] = l.y;
-    t.q[0] = make_float2(q.x, q.y);
-    t.q[1] = make_float2(q.z, q.w);
+    const u32x4 r = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rr, e * 16, 0, 0));
+    const u32x2 l = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(rl, e * 8, 0, 0));
+    const u32x4 q = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rq, e * 16, 0, 0));
+    t.r[0] = __builtin_bit_cast(double, (u32x2){r.x, r.y});
+    t.r[1] = __builtin_bit_cast(double, (u32x2){r.z, r.w});
+    t.l[0] = __uint_as_float(l.x);
+    t.l[1] = __uint_as_float(l.y);
+    t.q[0] = make_float2(__uint_as_float(q.x), __uint_as_float(q.y));
+    t.q[1] = make_float2(__uint_as_float(q.z), __uint_as_float(q.w));
   } else {
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
-      t.r[j] = (rt + row * NSUB)[e * NSUB + j];
-      t.l[j] = (la + row * NSUB)[e * NSUB + j];
-      t.q[j] = (qt + row * NSUB)[e * NSUB + j];
+      const int o = (e * NSUB + j);
+      t.r[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rr, o * 8, 0, 0));
+      t.l[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rl, o * 4, 0, 0));
+      const u32x2 q = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(rq, o * 8, 0, 0));
+      t.q[j] = make_float2(__uint_as_float(q.x), __uint_as_float(q.y));
     }
   }
   return t;
@@ -452,7 +464,7 @@ __device__ __forceinline__ void fetch_table(const float4* __restrict__ scat, con
     tt.valid[i] = s < s_end;
     const int sc = tt.valid[i] ? s : s_end - 1;  // in-bounds; masked in gen_fragments_table
     tt.wbits[i] = __float_as_int(scat[sc].z);
-    tt.t[i] = load_term<NSUB>(rt, la, qt, (size_t)sc * EP, e);
+    tt.t[i] = load_term<NSUB, EP>(rt, la, qt, (size_t)sc * EP, e);
   }
 }
 
@@ -524,6 +536,16 @@ struct WaveTiles {
     using L = FragLds<ET, FBK>;
     const int lane = threadIdx.x & 63;
     const u32x4 nim = {0u, 0u, 0x80008000u, 0x80008000u};
+    // Row operands are read from LDS in the shape they are used in: as stored
+    // (then negated in place) and with halves swapped (two 8-byte volatile
+    // reads, which the compiler does not merge, straight into one register
+    // quad), instead of one read plus register moves. The byte offset of the
+    // negated read passes through an empty asm statement so that the compiler
+    // cannot prove it equal to the column reads and share one read.
+    unsigned o_neg = (unsigned)lane * 16u;
+    asm volatile("" : "+v"(o_neg));
+    const unsigned o_zw = (unsigned)lane * 16u + 8u, o_xy = (unsigned)lane * 16u;
+    const char* fbytes = (const char*)frag;
 #pragma unroll
     for (int kk = 0; kk < KPW; ++kk) {
       const int k = k0 + kk;
@@ -531,14 +553,15 @@ struct WaveTiles {
 #pragma unroll
       for (int r = 0; r < ET; ++r) {
         if (!uses_row<ET, W>(r)) continue;
-        const u32x4 h = frag[L::idx(k, 0, r, lane)];
-        const u32x4 l = frag[L::idx(k, 1, r, lane)];
-        arh[r] = h ^ nim;
-        arl[r] = l ^ nim;
-        // halves swapped as 64-bit moves
-        const u64x2 h64 = __builtin_bit_cast(u64x2, h), l64 = __builtin_bit_cast(u64x2, l);
-        aih[r] = __builtin_bit_cast(u32x4, (u64x2){h64.y, h64.x});
-        ail[r] = __builtin_bit_cast(u32x4, (u64x2){l64.y, l64.x});
+        const unsigned eh = (unsigned)L::idx(k, 0, r, 0) * 16u, el = (unsigned)L::idx(k, 1, r, 0) * 16u;
+        arh[r] = *(const u32x4*)(fbytes + o_neg + eh) ^ nim;
+        arl[r] = *(const u32x4*)(fbytes + o_neg + el) ^ nim;
+        const u32x2 hzw = *(const volatile lds_u32x2*)(fbytes + o_zw + eh);
+        const u32x2 hxy = *(const volatile lds_u32x2*)(fbytes + o_xy + eh);
+        const u32x2 lzw = *(const volatile lds_u32x2*)(fbytes + o_zw + el);
+        const u32x2 lxy = *(const volatile lds_u32x2*)(fbytes + o_xy + el);
+        aih[r] = (u32x4){hzw.x, hzw.y, hxy.x, hxy.y};
+        ail[r] = (u32x4){lzw.x, lzw.y, lxy.x, lxy.y};
       }
 #pragma unroll
       for (int c = 0; c < ET; ++c) {
