@@ -146,7 +146,7 @@ def bench_phantoms(args) -> int:
     prob = B._problem(pts, rcs, delays, None, plan, params, medium, False, dev)
     from fastsimus_amd import tune
 
-    tuning = None if args.no_tune else tune.autotune(
+    tuning = None if not args.tune else tune.autotune(
         prob, off, reps=1, candidates=[{"msm_fbk": 4, "msm_item_div": d} for d in (0, 4, 16)])
     ws = K.batch_workspace(prob, off, "mfma")
     nf, E, T, P, S = prob.n_freq, params.n_elements, prob.n_tx, len(phantoms), int(off[-1])
@@ -366,7 +366,9 @@ def main() -> int:
                          "array, focused beams), adjoint15 (forward + gradient of the differentiable form) or "
                          "pfield (RMS pressure field on a grid)")
     ap.add_argument("--n-phantoms", type=int, default=64)
-    ap.add_argument("--no-tune", action="store_true", help="skip the tile-shape autotuner (library defaults)")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the tile-shape autotuner before timing (default: library defaults, so that a kernel "
+                         "trace of the default command holds only the timed shape)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (calibrates the FETCH_SIZE / WRITE_SIZE "
                          "counters of a profiler against a known byte count) and exit")
@@ -413,7 +415,7 @@ def main() -> int:
     lib = _lib.load()
     from fastsimus_amd import tune
 
-    tuning = tune.autotune(prob, reps=1) if not args.no_tune else None
+    tuning = tune.autotune(prob, reps=1) if args.tune else None
 
     def step():
         sp = K.run_spectrum(prob, "mfma")
