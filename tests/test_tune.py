@@ -96,3 +96,29 @@ def test_autotune_single_and_batch(cuda):
     after = B.simus_batch_compute(ph, D, plan, p, m).rf.cpu().numpy()
     assert rel_l2(after, before) < 1e-5
     tune.set_tuning(bprob, 3)
+
+
+@pytest.mark.parametrize("case", ["convex_c5", "impedance_nsub3", "l11_speckle_atten"])
+def test_geometry_table_matches_per_block_geometry(case, cuda):
+    """The frequency-independent geometry read from the per-call table (default) and evaluated
+    per frequency block (msm_table = 2) agree to FP32 rounding of the amplitudes, and both meet
+    the FP64 oracle: convex array, impedance baffle with three sub-elements, soft baffle."""
+    from tests.test_gpu_simus import CASES, _oracle_rf
+
+    params, medium, x, z, rc, D, kw = CASES[case]()
+    D = np.atleast_2d(D)
+    sc = np.stack([x, z], -1)
+    plan = fs.simus_precompute(sc, rc, D, params, medium, element_splitting=kw.get("element_splitting"))
+    prob = K.Problem(sc, rc, D, None, params, medium, n_sub=plan.n_sub, seg_length=plan.seg_length,
+                     freqs=plan.selected_freqs, pulse=plan.pulse_spectrum, probe=plan.probe_spectrum,
+                     correction=plan.correction_factor, full_frequency_directivity=False, device=cuda)
+    print(case, flush=True)
+    tune.set_tuning(prob)
+    with_table = fs.simus_compute(sc, rc, D, plan, params, medium, strategy="hip_mfma").rf.cpu().numpy()
+    tune.set_tuning(prob, msm_table=2)
+    per_block = fs.simus_compute(sc, rc, D, plan, params, medium, strategy="hip_mfma").rf.cpu().numpy()
+    tune.set_tuning(prob)
+    want = _oracle_rf(x, z, rc, D, params, medium, **kw)
+    assert rel_l2(with_table, per_block) < 3e-5
+    assert rel_l2(with_table, want) < 1e-3
+    assert rel_l2(per_block, want) < 1e-3
