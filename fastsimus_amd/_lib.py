@@ -35,6 +35,7 @@ EXPORTS = (
     "fs_simus_adjoint",
     "fs_pfield_workspace_bytes",
     "fs_pfield",
+    "fs_max_distance",
     "fs_set_kernel_timing",
     "fs_kernel_time_ms",
 )
@@ -94,7 +95,7 @@ def load() -> C.CDLL:
     lib.fs_tuning_get.restype = C.c_int
     lib.fs_tuning_get.argtypes = [D, C.c_int, C.POINTER(Tuning)]
     lib.fs_rf_workspace_bytes.restype = C.c_size_t
-    lib.fs_rf_workspace_bytes.argtypes = [C.c_int]
+    lib.fs_rf_workspace_bytes.argtypes = [C.c_int, C.c_int, C.c_int]
     lib.fs_spectrum_to_rf.restype = C.c_int
     lib.fs_spectrum_to_rf.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P, P,
                                       C.c_size_t, P]
@@ -110,6 +111,8 @@ def load() -> C.CDLL:
     lib.fs_pfield_workspace_bytes.argtypes = [D]
     lib.fs_pfield.restype = C.c_int
     lib.fs_pfield.argtypes = [D, P, P, C.c_size_t, P]
+    lib.fs_max_distance.restype = C.c_int
+    lib.fs_max_distance.argtypes = [P, P, C.c_longlong, P, P, C.c_int, P, P]
     lib.fs_set_kernel_timing.restype = C.c_int
     lib.fs_set_kernel_timing.argtypes = [C.c_int]
     lib.fs_kernel_time_ms.restype = C.c_double

@@ -17,8 +17,9 @@ int mfma_spectrum_batch(const Problem& p, int P, const int64_t* off, cf* spect, 
 size_t mfma_workspace_batch(const Problem& p, int P, const int64_t* off);
 fs_tuning msm_tuning(const Problem& p, int P);
 int msm_tuning_set(const Problem& p, int P, const fs_tuning* t);
+size_t rf_workspace(int NT, int NF, int N);
 int spectrum_to_rf(const cf* spect, int NT, int NF, int E, int nff, int idx0, int N, cf* full, float* rf, float* rf_u,
-                   unsigned* peak, hipStream_t st);
+                   char* ws, hipStream_t st);
 size_t rf_adjoint_ws(int NT, int nkeep, int E);
 int rf_adjoint(const float* u, const float* dy, int NT, int NF, int E, int idx0, int N, cf* dspect, char* ws,
                hipStream_t st);
@@ -205,16 +206,18 @@ int fs_tuning_get(const fs_simus_desc* d, int n_phantoms, fs_tuning* t) {
   return FS_OK;
 }
 
-size_t fs_rf_workspace_bytes(int n_tx) { return n_tx > 0 ? ((size_t)n_tx * sizeof(unsigned) + 255) / 256 * 256 : 0; }
+size_t fs_rf_workspace_bytes(int n_tx, int n_freq, int n_fft) {
+  return (n_tx > 0 && n_freq > 0 && n_fft > 0) ? rf_workspace(n_tx, n_freq, n_fft) : 0;
+}
 
 int fs_spectrum_to_rf(const float* spect, int n_tx, int n_freq, int n_elem, int n_ff, int idx0, int n_fft,
                       float* spectrum_full, float* rf, float* rf_ungated, void* workspace, size_t workspace_bytes,
                       void* stream) {
   if (!spect || !rf || n_tx <= 0 || n_freq <= 0 || n_elem <= 0 || n_fft <= 0 || idx0 < 0) return FS_ERR_INVALID;
   if (spectrum_full && (n_ff <= 0 || idx0 + n_freq > n_ff)) return FS_ERR_INVALID;
-  if (!workspace || workspace_bytes < fs_rf_workspace_bytes(n_tx)) return FS_ERR_WORKSPACE;
+  if (!workspace || workspace_bytes < fs_rf_workspace_bytes(n_tx, n_freq, n_fft)) return FS_ERR_WORKSPACE;
   return spectrum_to_rf((const cf*)spect, n_tx, n_freq, n_elem, n_ff, idx0, n_fft, (cf*)spectrum_full, rf, rf_ungated,
-                        (unsigned*)workspace, (hipStream_t)stream);
+                        (char*)workspace, (hipStream_t)stream);
 }
 
 size_t fs_rf_adjoint_workspace_bytes(int n_tx, int n_keep, int n_elem) {

@@ -8,10 +8,13 @@
 //
 // (irfft of conj(X): bins above N/2 are cropped, the imaginary parts of the DC
 // and Nyquist bins are ignored, w_k = 1 at DC / Nyquist and 2 elsewhere).
-// A 64(n) x 32(e) output tile per workgroup: the band is staged through LDS in
-// chunks of 64 bins, all lanes of a wave read the same 8 elements (LDS
-// broadcast), and each lane advances its own twiddle exp(i 2 pi n / N) by an
-// FP32 recurrence reseeded from an exact integer (k n mod N) at every chunk.
+// For every transmit this is one real matrix product U (n x E) = W (n x 2 NF)
+// Y (2 NF x E) with W[n][2k] = w cos / N, W[n][2k+1] = w sin / N and Y the real
+// and imaginary parts of the band, which runs on the matrix cores in split
+// BF16 (hi + lo, hi*hi + hi*lo + lo*hi, FP32 accumulation): k_idft_table
+// writes W once per call in MFMA fragment order (the phase b n mod N formed
+// exactly in integers), k_idft_mfma stages 8 row tiles of it per 16-bin chunk
+// in LDS (double-buffered) and splits the spectrum's fragments in registers.
 // The -100 dB smooth gate needs max|u| per transmit: every workgroup folds its
 // tile maximum into one word with an integer atomicMax on the float bits
 // (order independent, so the result is deterministic).
@@ -67,61 +70,143 @@ __device__ __forceinline__ cf twiddle(long m, int N) {
   return {c, s};
 }
 
-__global__ __launch_bounds__(256) void k_idft(const cf* __restrict__ spect, int NF, int E, int idx0, int N, int nkeep,
-                                              float* __restrict__ rf, unsigned* __restrict__ peak) {
-  __shared__ cf xs[RK][RE];
-  __shared__ float wmax[4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int n = blockIdx.x * RT + lane;
-  const int e0 = blockIdx.y * RE;
-  const int t = blockIdx.z;
-  const cf* X = spect + (size_t)t * NF * E;
-  const float invN = 1.0f / (float)N;
-  const cf st = twiddle(n % N, N);
-  float acc[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) acc[j] = 0.f;
+typedef __bf16 rf_bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 rf_bf16x2 __attribute__((ext_vector_type(2)));
+typedef float rf_floatx4 __attribute__((ext_vector_type(4)));
+typedef float rf_floatx2 __attribute__((ext_vector_type(2)));
+typedef unsigned rf_u32x4 __attribute__((ext_vector_type(4)));
 
-  for (int i0 = 0; i0 < NF; i0 += RK) {
-    __syncthreads();
-    for (int q = threadIdx.x; q < RK * RE; q += 256) {
-      const int kk = q / RE, ee = q % RE;
-      const int i = i0 + kk, e = e0 + ee;
-      cf v = {0.f, 0.f};
-      if (i < NF && e < E) {
-        const float w = bin_weight(idx0 + i, N) * invN;
-        const cf x = X[(size_t)i * E + e];
-        v = {x.x * w, x.y * w};
-      }
-      xs[kk][ee] = v;
-    }
-    __syncthreads();
-    cf tw = twiddle(((long)(idx0 + i0) * n) % N, N);
-    const int kmax = min(RK, NF - i0);
-    for (int kk = 0; kk < kmax; ++kk) {
-      const cf* row = &xs[kk][wave * 8];
+constexpr int IT_N = 8;  // row tiles (16 output samples each) per workgroup
+constexpr int IK = 16;   // bins per K chunk (32 real products per MFMA)
+
+// a, b -> packed bf16 hi (round to nearest) and the bf16 of the remainders
+__device__ __forceinline__ void rf_split(float a, float b, unsigned& hi, unsigned& lo) {
+  const rf_bf16x2 h = __builtin_convertvector((rf_floatx2){a, b}, rf_bf16x2);
+  const unsigned hb = __builtin_bit_cast(unsigned, h);
+  const float ah = __uint_as_float(hb << 16), bh = __uint_as_float(hb & 0xffff0000u);
+  const rf_bf16x2 l = __builtin_convertvector((rf_floatx2){a - ah, b - bh}, rf_bf16x2);
+  hi = hb;
+  lo = __builtin_bit_cast(unsigned, l);
+}
+
+__device__ __forceinline__ rf_floatx4 rf_mma(rf_u32x4 a, rf_u32x4 b, rf_floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(rf_bf16x8, a), __builtin_bit_cast(rf_bf16x8, b), c,
+                                                 0, 0, 0);
+}
+
+// W fragments [nt][kc][hl][lane] (A operand of v_mfma_f32_16x16x32_bf16: lane -> row lane % 16,
+// K slots 8 (lane / 16) + 0..7): row n = 16 nt + lane % 16, bins b = idx0 + 16 kc + 4 (lane / 16) + q
+// (q = 0..3), slots (w_b / N) (cos, sin)(2 pi b n / N); zero for n >= nkeep, beyond the band and
+// above N / 2.
+__global__ void k_idft_table(int NF, int idx0, int N, int nkeep, int ntp, int nkc, rf_u32x4* __restrict__ W) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= (long)ntp * nkc * 64) return;
+  const int lane = (int)(i & 63);
+  const long r = i >> 6;  // nt * nkc + kc
+  const int kc = (int)(r % nkc), nt = (int)(r / nkc);
+  const int n = 16 * nt + (lane & 15);
+  float v[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) acc[j] = fmaf(row[j].x, tw.x, fmaf(row[j].y, tw.y, acc[j]));
-      tw = cmul(tw, st);
+  for (int q = 0; q < 4; ++q) {
+    const int k = IK * kc + 4 * (lane >> 4) + q;
+    const int b = idx0 + k;
+    float c = 0.f, sn = 0.f;
+    if (n < nkeep && k < NF && 2 * b <= N) {
+      const float w = bin_weight(b, N) / (float)N;
+      const cf tw = twiddle(((long)b * n) % N, N);
+      c = w * tw.x;
+      sn = w * tw.y;
     }
+    v[2 * q] = c;
+    v[2 * q + 1] = sn;
+  }
+  unsigned h[4], l[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) rf_split(v[2 * q], v[2 * q + 1], h[q], l[q]);
+  W[(r * 2 + 0) * 64 + lane] = (rf_u32x4){h[0], h[1], h[2], h[3]};
+  W[(r * 2 + 1) * 64 + lane] = (rf_u32x4){l[0], l[1], l[2], l[3]};
+}
+
+// u[t][n][e] for one transmit t (blockIdx.z), IT_N row tiles (blockIdx.x) and NW element tiles of 16
+// (wave w: elements 16 (NW blockIdx.y + w) + 0..15). B operand: lane -> element lane % 16, bins
+// 16 kc + 4 (lane / 16) + 0..3 as (Re, Im) pairs, matching W's slots.
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_idft_mfma(const cf* __restrict__ spect, int NF, int E, int nkeep, int nkc,
+                                                       const rf_u32x4* __restrict__ W, float* __restrict__ u,
+                                                       unsigned* __restrict__ peak) {
+  constexpr int ENT = IT_N * 2 * 64;   // W fragments per chunk per workgroup
+  constexpr int PER = ENT / (64 * NW);  // of them per thread
+  __shared__ rf_u32x4 wa[2][ENT];
+  __shared__ float wmax[NW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nt0 = blockIdx.x * IT_N;
+  const int e = (blockIdx.y * NW + wave) * 16 + (lane & 15);
+  const int t = blockIdx.z;
+  const bool ev = e < E;
+  const cf* X = spect + (size_t)t * NF * E + (ev ? e : 0);
+  const rf_u32x4* Wb = W + (size_t)nt0 * nkc * 128;
+  rf_u32x4 pre[PER];
+  auto fetch = [&](int kc) {
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int q = threadIdx.x + j * 64 * NW;  // (row tile q / 128, hi|lo and lane q % 128)
+      pre[j] = Wb[((size_t)(q >> 7) * nkc + kc) * 128 + (q & 127)];
+    }
+  };
+  auto stash = [&](int b) {
+#pragma unroll
+    for (int j = 0; j < PER; ++j) wa[b][threadIdx.x + j * 64 * NW] = pre[j];
+  };
+  rf_floatx4 acc[IT_N];
+#pragma unroll
+  for (int i = 0; i < IT_N; ++i) acc[i] = (rf_floatx4){0.f, 0.f, 0.f, 0.f};
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (int kc = 0; kc < nkc; ++kc) {
+    const int b = kc & 1;
+    if (kc + 1 < nkc) fetch(kc + 1);
+    float y[8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = IK * kc + 4 * (lane >> 4) + q;
+      const cf x = (ev && k < NF) ? X[(size_t)k * E] : cf{0.f, 0.f};
+      y[2 * q] = x.x;
+      y[2 * q + 1] = x.y;
+    }
+    unsigned h[4], l[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) rf_split(y[2 * q], y[2 * q + 1], h[q], l[q]);
+    const rf_u32x4 bh = {h[0], h[1], h[2], h[3]}, bl = {l[0], l[1], l[2], l[3]};
+#pragma unroll
+    for (int i = 0; i < IT_N; ++i) {
+      const rf_u32x4 ah = wa[b][(i * 2 + 0) * 64 + lane], al = wa[b][(i * 2 + 1) * 64 + lane];
+      acc[i] = rf_mma(ah, bh, acc[i]);
+      acc[i] = rf_mma(ah, bl, acc[i]);
+      acc[i] = rf_mma(al, bh, acc[i]);
+    }
+    if (kc + 1 < nkc) stash(b ^ 1);  // the other buffer: every wave finished reading it before the last barrier
+    __syncthreads();
   }
   float m = 0.f;
-  if (n < nkeep) {
-    float* out = rf + ((size_t)t * nkeep + n) * E;
+  if (ev) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int e = e0 + wave * 8 + j;
-      if (e < E) {
-        out[e] = acc[j];
-        m = fmaxf(m, fabsf(acc[j]));
+    for (int i = 0; i < IT_N; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = 16 * (nt0 + i) + 4 * (lane >> 4) + r;
+        if (n < nkeep) {
+          u[((size_t)t * nkeep + n) * E + e] = acc[i][r];
+          m = fmaxf(m, fabsf(acc[i][r]));
+        }
       }
-    }
   }
   for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
   if (lane == 0) wmax[wave] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
-    const float b = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+    float b = 0.f;
+    for (int w = 0; w < NW; ++w) b = fmaxf(b, wmax[w]);
     atomicMax(peak + t, __float_as_uint(b));
   }
 }
@@ -241,9 +326,28 @@ __global__ __launch_bounds__(256) void k_dft_adj(const float* __restrict__ du, i
 }
 
 // ---------------------------------------------------------------- host side
+// Workspace of spectrum_to_rf: the per-transmit peak words, then the W fragments.
+struct RfLayout {
+  int nkeep, ntp, nkc;
+  size_t w_offset, bytes;
+};
+static RfLayout rf_layout(int NT, int NF, int N) {
+  RfLayout L;
+  L.nkeep = (N + 1) / 2;
+  L.ntp = (L.nkeep + 16 * IT_N - 1) / (16 * IT_N) * IT_N;
+  L.nkc = (NF + IK - 1) / IK;
+  L.w_offset = ((size_t)NT * sizeof(unsigned) + 255) / 256 * 256;
+  L.bytes = L.w_offset + (size_t)L.ntp * L.nkc * 128 * sizeof(rf_u32x4);
+  return L;
+}
+size_t rf_workspace(int NT, int NF, int N) { return rf_layout(NT, NF, N).bytes; }
+
 int spectrum_to_rf(const cf* spect, int NT, int NF, int E, int nff, int idx0, int N, cf* full, float* rf,
-                   float* rf_u, unsigned* peak, hipStream_t st) {
-  const int nkeep = (N + 1) / 2;
+                   float* rf_u, char* ws, hipStream_t st) {
+  const RfLayout L = rf_layout(NT, NF, N);
+  const int nkeep = L.nkeep;
+  unsigned* peak = (unsigned*)ws;
+  rf_u32x4* W = (rf_u32x4*)(ws + L.w_offset);
   hipLaunchKernelGGL(k_fill_u32, dim3((NT + 255) / 256), dim3(256), 0, st, peak, NT, 0u);
   if (full) {
     const long n = (long)NT * nff * E;
@@ -251,8 +355,18 @@ int spectrum_to_rf(const cf* spect, int NT, int NF, int E, int nff, int idx0, in
                        nff, idx0, full);
   }
   float* u = rf_u ? rf_u : rf;
-  dim3 grid((nkeep + RT - 1) / RT, (E + RE - 1) / RE, NT);
-  hipLaunchKernelGGL(k_idft, grid, dim3(256), 0, st, spect, NF, E, idx0, N, nkeep, u, peak);
+  {
+    const long n = (long)L.ntp * L.nkc * 64;
+    hipLaunchKernelGGL(k_idft_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, NF, idx0, N, nkeep, L.ntp,
+                       L.nkc, W);
+  }
+  if (E > 64) {
+    dim3 grid(L.ntp / IT_N, (E + 127) / 128, NT);
+    hipLaunchKernelGGL(k_idft_mfma<8>, grid, dim3(512), 0, st, spect, NF, E, nkeep, L.nkc, W, u, peak);
+  } else {
+    dim3 grid(L.ntp / IT_N, (E + 63) / 64, NT);
+    hipLaunchKernelGGL(k_idft_mfma<4>, grid, dim3(256), 0, st, spect, NF, E, nkeep, L.nkc, W, u, peak);
+  }
   const long n = (long)nkeep * E * NT;
   hipLaunchKernelGGL(k_gate, dim3((int)std::min<long>((n + 255) / 256, 65536)), dim3(256), 0, st, u, peak, nkeep * E,
                      NT, rf);

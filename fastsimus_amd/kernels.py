@@ -188,7 +188,7 @@ def run_rf(spect: torch.Tensor, n_ff: int, idx0: int, n_fft: int, *, want_full: 
     rf = torch.empty((T, nkeep, E), dtype=torch.float32, device=dev)
     full = torch.empty((T, n_ff, E, 2), dtype=torch.float32, device=dev) if want_full else None
     ung = torch.empty((T, nkeep, E), dtype=torch.float32, device=dev) if want_ungated else None
-    ws = _workspace(lib.fs_rf_workspace_bytes(T), dev)
+    ws = _workspace(lib.fs_rf_workspace_bytes(T, NF, n_fft), dev)
     st = lib.fs_spectrum_to_rf(s.data_ptr(), T, NF, E, n_ff, idx0, n_fft, _ptr(full), rf.data_ptr(), _ptr(ung),
                                ws.data_ptr(), ws.numel(), _stream(dev))
     _lib.check(st, "fs_spectrum_to_rf")

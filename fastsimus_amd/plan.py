@@ -123,12 +123,32 @@ def _as_xz_f64(points):
     return p[:, 0], p[:, 1]
 
 
+def _max_distance_hip(x, z, ex: np.ndarray, ez: np.ndarray) -> float:
+    """Device points: one pass of the library's FP64 reduction (fs_max_distance), one synchronisation."""
+    from . import _lib
+
+    lib = _lib.load()
+    dev = x.device
+    with torch.cuda.device(dev):
+        xs = x.contiguous()
+        zs = z.contiguous()
+        sx = torch.as_tensor(np.ascontiguousarray(ex.reshape(-1)), dtype=torch.float64, device=dev)
+        sz = torch.as_tensor(np.ascontiguousarray(ez.reshape(-1)), dtype=torch.float64, device=dev)
+        out = torch.empty(1, dtype=torch.float64, device=dev)
+        st = lib.fs_max_distance(xs.data_ptr(), zs.data_ptr(), xs.numel(), sx.data_ptr(), sz.data_ptr(), sx.numel(),
+                                 out.data_ptr(), torch.cuda.current_stream(dev).cuda_stream)
+        _lib.check(st, "fs_max_distance")
+        return float(out.item())
+
+
 def _max_distance(x, z, ex: np.ndarray, ez: np.ndarray, clip: float | None = None) -> float:
     """max over points and sources of the Euclidean distance (optionally clipped below)."""
     best = 0.0
     n = x.shape[0]
     chunk = max(1, (1 << 22) // max(1, ex.size))
-    if torch is not None and isinstance(x, torch.Tensor):
+    if torch is not None and isinstance(x, torch.Tensor) and x.is_cuda:
+        best = _max_distance_hip(x, z, ex, ez)
+    elif torch is not None and isinstance(x, torch.Tensor):
         tex = torch.as_tensor(ex.reshape(-1), dtype=torch.float64, device=x.device)
         tez = torch.as_tensor(ez.reshape(-1), dtype=torch.float64, device=x.device)
         for i in range(0, n, chunk):

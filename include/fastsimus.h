@@ -25,6 +25,8 @@
  *   fs_simus_spectrum_batch <- a loop of simus_cuda calls over phantoms (no batched API in the
  *                            reference; its harness loops, tests/_picmus_phantom_shared.py:319-367)
  *   fs_tuning_set / get   <- the fixed tuning constants of kernels/cuda_simus.py:39-45
+ *   fs_max_distance       <- the O(points x sources) distance maximum of the plans,
+ *                            src/fast_simus/simus.py:219-222 and pfield.py:271-282
  */
 #ifndef FASTSIMUS_H
 #define FASTSIMUS_H
@@ -140,9 +142,10 @@ int fs_tuning_get(const fs_simus_desc* d, int n_phantoms, fs_tuning* t);
 /* Place the band into the full spectrum and compute gated RF per transmit:
  *   spectrum_full[t][i][e] (complex64, i < n_ff), zero outside [idx0, idx0 + n_freq);
  *   rf[t][n][e] = gate(irfft(conj(full), n_fft)[n]), n < (n_fft + 1) / 2.
- * workspace: >= fs_rf_workspace_bytes(n_tx). May be called with spectrum_full == NULL.
+ * workspace: >= fs_rf_workspace_bytes(n_tx, n_freq, n_fft) (the per-transmit peaks and the split-BF16
+ * inverse-DFT matrix, (n_fft + 1) / 2 rows x 2 n_freq columns). May be called with spectrum_full == NULL.
  * rf_ungated, when not NULL, receives the pre-gate RF (needed by fs_rf_adjoint). */
-size_t fs_rf_workspace_bytes(int n_tx);
+size_t fs_rf_workspace_bytes(int n_tx, int n_freq, int n_fft);
 int fs_spectrum_to_rf(const float* spect, int n_tx, int n_freq, int n_elem, int n_ff, int idx0, int n_fft,
                       float* spectrum_full, float* rf, float* rf_ungated, void* workspace, size_t workspace_bytes,
                       void* stream);
@@ -165,6 +168,13 @@ int fs_simus_adjoint(const fs_simus_desc* d, const float* d_spect, float* d_rc, 
  *   out[p] = sqrt(correction * sum_k |pulse_k probe_k sum_{e,j} g A|^2), 0 where out of field. */
 size_t fs_pfield_workspace_bytes(const fs_simus_desc* d);
 int fs_pfield(const fs_simus_desc* d, float* out, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Largest distance from any of n_points points (x, z) to any of n_src sources (src_x, src_z),
+ * all float64 device arrays, written to out[0] (device, float64). The frequency plans use it
+ * (simus: element centres; pfield: sub-element centroids). One pass, no host synchronisation:
+ * the caller reads out[0] when it needs the plan. n_points may be 0 (out = 0). */
+int fs_max_distance(const double* x, const double* z, long long n_points, const double* src_x, const double* src_z,
+                    int n_src, double* out, void* stream);
 
 /* Optional timing of the dominant kernel launches (k_msm for FS_METHOD_MFMA_BF16X3, k_tx + k_rx for
  * FS_METHOD_FP32): while enabled, each such launch is bracketed by HIP events on its stream.

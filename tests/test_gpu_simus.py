@@ -262,3 +262,31 @@ def test_ragged_element_counts(n_elem, cuda):
     else:
         rfm, _ = _run("hip_mfma", x, z, rc, D, p, m)
         assert rel_l2(rfm, want) < MFMA_TOL
+
+
+def test_device_plan_matches_host_plan(cuda):
+    """simus_precompute / pfield_precompute on device tensors (one fs_max_distance pass) give the
+    plans the host (NumPy FP64) computation gives; the reduction itself matches NumPy in FP64."""
+    import torch
+
+    from fastsimus_amd.plan import _max_distance
+
+    x, z, rc = speckle(11, 5003)
+    params, m = fs.L11_5v(), fs.MediumParams(attenuation=0.5)
+    pos = _pos(params)
+    D = np.stack([fs.plane_wave(pos, np.deg2rad(a), speed_of_sound=1540.0) for a in (-8.0, 8.0)])
+    sc = np.stack([x, z], -1)
+    host = fs.simus_precompute(sc, rc, D, params, m)
+    dev = fs.simus_precompute(torch.as_tensor(sc, device=cuda), rc, D, params, m)
+    assert (host.n_freq_full, host.freq_idx_start, host.n_fft, host.n_sub) == \
+        (dev.n_freq_full, dev.freq_idx_start, dev.n_fft, dev.n_sub)
+    np.testing.assert_allclose(dev.selected_freqs, host.selected_freqs, rtol=1e-12)
+    xd = torch.as_tensor(x, dtype=torch.float64, device=cuda)
+    zd = torch.as_tensor(z, dtype=torch.float64, device=cuda)
+    want = float(np.sqrt((x[:, None] - pos[:, 0]) ** 2 + (z[:, None] - pos[:, 1]) ** 2).max())
+    assert abs(_max_distance(xd, zd, pos[:, 0], pos[:, 1]) - want) <= 1e-14 * want
+    assert _max_distance(xd[:0], zd[:0], pos[:, 0], pos[:, 1]) == 0.0
+    grid = np.stack(np.meshgrid(np.linspace(-0.02, 0.02, 41), np.linspace(0.0, 0.04, 37)), -1).reshape(-1, 2)
+    ph = fs.pfield_precompute(grid, D[0], params, m)
+    pd = fs.pfield_precompute(torch.as_tensor(grid, device=cuda), D[0], params, m)
+    np.testing.assert_allclose(pd.selected_freqs, ph.selected_freqs, rtol=1e-12)
