@@ -77,6 +77,7 @@ typedef float rf_floatx2 __attribute__((ext_vector_type(2)));
 typedef unsigned rf_u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int IT_N = 8;  // row tiles (16 output samples each) per workgroup
+constexpr int kMaxGridZ = 65535;  // transmits per launch (grid z)
 constexpr int IK = 16;   // bins per K chunk (32 real products per MFMA)
 
 // a, b -> packed bf16 hi (round to nearest) and the bf16 of the remainders
@@ -360,12 +361,18 @@ int spectrum_to_rf(const cf* spect, int NT, int NF, int E, int nff, int idx0, in
     hipLaunchKernelGGL(k_idft_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, NF, idx0, N, nkeep, L.ntp,
                        L.nkc, W);
   }
-  if (E > 64) {
-    dim3 grid(L.ntp / IT_N, (E + 127) / 128, NT);
-    hipLaunchKernelGGL(k_idft_mfma<8>, grid, dim3(512), 0, st, spect, NF, E, nkeep, L.nkc, W, u, peak);
-  } else {
-    dim3 grid(L.ntp / IT_N, (E + 63) / 64, NT);
-    hipLaunchKernelGGL(k_idft_mfma<4>, grid, dim3(256), 0, st, spect, NF, E, nkeep, L.nkc, W, u, peak);
+  // transmits on grid z, in launches of at most kMaxGridZ
+  for (int t0 = 0; t0 < NT; t0 += kMaxGridZ) {
+    const int nt = std::min(kMaxGridZ, NT - t0);
+    const cf* sp = spect + (size_t)t0 * NF * E;
+    float* ut = u + (size_t)t0 * nkeep * E;
+    if (E > 64) {
+      dim3 grid(L.ntp / IT_N, (E + 127) / 128, nt);
+      hipLaunchKernelGGL(k_idft_mfma<8>, grid, dim3(512), 0, st, sp, NF, E, nkeep, L.nkc, W, ut, peak + t0);
+    } else {
+      dim3 grid(L.ntp / IT_N, (E + 63) / 64, nt);
+      hipLaunchKernelGGL(k_idft_mfma<4>, grid, dim3(256), 0, st, sp, NF, E, nkeep, L.nkc, W, ut, peak + t0);
+    }
   }
   const long n = (long)nkeep * E * NT;
   hipLaunchKernelGGL(k_gate, dim3((int)std::min<long>((n + 255) / 256, 65536)), dim3(256), 0, st, u, peak, nkeep * E,
@@ -393,13 +400,22 @@ int rf_adjoint(const float* u, const float* dy, int NT, int NF, int E, int idx0,
   // recompute the peak of the pre-gate RF (max of |u| per transmit)
   {
     const int nblk = 256;
-    dim3 g(nblk, NT);
-    hipLaunchKernelGGL(k_absmax, g, dim3(256), 0, st, u, per, peak);
-    hipLaunchKernelGGL(k_gate_adj1, g, dim3(256), 0, st, u, dy, peak, per, du, argm, psum);
+    for (int t0 = 0; t0 < NT; t0 += kMaxGridZ) {  // transmits on grid y, in launches of at most kMaxGridZ
+      const int nt = std::min(kMaxGridZ, NT - t0);
+      const size_t o = (size_t)t0 * per;
+      dim3 g(nblk, nt);
+      hipLaunchKernelGGL(k_absmax, g, dim3(256), 0, st, u + o, per, peak + t0);
+      hipLaunchKernelGGL(k_gate_adj1, g, dim3(256), 0, st, u + o, dy + o, peak + t0, per, du + o, argm + t0,
+                         psum + (size_t)t0 * nblk);
+    }
     hipLaunchKernelGGL(k_gate_adj2, dim3((NT + 255) / 256), dim3(256), 0, st, u, argm, psum, nblk, per, NT, du);
   }
-  dim3 grid((NF + RT - 1) / RT, (E + RE - 1) / RE, NT);
-  hipLaunchKernelGGL(k_dft_adj, grid, dim3(256), 0, st, du, NF, E, idx0, N, nkeep, dspect);
+  for (int t0 = 0; t0 < NT; t0 += kMaxGridZ) {
+    const int nt = std::min(kMaxGridZ, NT - t0);
+    dim3 grid((NF + RT - 1) / RT, (E + RE - 1) / RE, nt);
+    hipLaunchKernelGGL(k_dft_adj, grid, dim3(256), 0, st, du + (size_t)t0 * nkeep * E, NF, E, idx0, N, nkeep,
+                       dspect + (size_t)t0 * NF * E);
+  }
   return hipGetLastError() == hipSuccess ? FS_OK : FS_ERR_LAUNCH;
 }
 
