@@ -253,12 +253,14 @@ __global__ void k_scat_prep(Problem p, float4* __restrict__ out) {
 // Frequency-independent part of every (scatterer, element, sub-element) term,
 // evaluated once per call in FP64 (reference math: _pfield_math.py:71-247):
 //   rt = r_c = max(r, c / (2 fc))                         (m, FP64)
-//   la = log2(obliquity(theta) directivity(theta) |w| / sqrt(r_c))
+//   la = (log2(obliquity(theta) directivity(theta) |w| / sqrt(r_c)), float(r_c))
 //   qt = exp((-att + i 2 pi / c) r_c df)                  (the per-frequency step)
 // with w = sqrt|rc| / n_sub from k_scat_prep (the sign of rc is applied in
 // k_msm), directivity at the centre frequency, theta = asin(dx / r) - theta_e
 // (pi/2 at r = 0). la = -inf for dead scatterers and padded elements, whose
-// rt is left finite (c / (2 fc)) so that their terms are exactly zero.
+// rt is left finite (c / (2 fc)) so that their terms are exactly zero; row
+// s = S is such a dead row for every element, read by k_msm in place of the
+// scatterers that pad the last tile of a unit.
 // Layout [s][e][j] with EP (padded element count) elements per scatterer, so
 // that a lane of k_msm reads its NSUB terms of one scatterer as one vector load.
 // Requires a non-negative amplitude: sinc(kappa sin(theta)) >= 0 (kappa <= pi)
@@ -267,10 +269,10 @@ template <int NSUB>
 __global__ void k_geo_table(Problem p, const float4* __restrict__ scat, int EP, double* __restrict__ rt,
                             float* __restrict__ la, float2* __restrict__ qt) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)p.S * EP) return;
+  if (idx >= (long long)(p.S + 1) * EP) return;
   const int s = (int)(idx / EP);
   const int e = (int)(idx - (long long)s * EP);
-  const float4 sd = scat[s];
+  const float4 sd = s < p.S ? scat[s] : make_float4(0.f, 1.f, 0.f, 0.f);  // row S: dead
   const double w = fabs((double)sd.z);
 #pragma unroll
   for (int j = 0; j < NSUB; ++j) {
@@ -296,7 +298,8 @@ __global__ void k_geo_table(Problem p, const float4* __restrict__ scat, int EP, 
     sincospi(2.0 * (p.df * p.inv_c * rcl), &qs, &qc);
     const double qm = exp(-(double)p.att_coef * p.df * rcl);
     rt[idx * NSUB + j] = rcl;
-    la[idx * NSUB + j] = (float)l;
+    la[2 * (idx * NSUB + j)] = (float)l;
+    la[2 * (idx * NSUB + j) + 1] = (float)rcl;
     qt[idx * NSUB + j] = make_float2((float)(qm * qc), (float)(qm * qs));
   }
 }
@@ -401,7 +404,7 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<N
 template <int NSUB>
 struct TableTerm {
   double r[NSUB];
-  float l[NSUB];
+  float l[NSUB], rf[NSUB];
   float2 q[NSUB];
 };
 // Buffer resource over one table row (wave-uniform base, the row's size in
@@ -418,16 +421,18 @@ __device__ __forceinline__ TableTerm<NSUB> load_term(const double* __restrict__ 
                                                      const float2* __restrict__ qt, size_t row, int e) {
   TableTerm<NSUB> t;
   const __amdgpu_buffer_rsrc_t rr = row_rsrc(rt + row * NSUB, EP * NSUB * 8);
-  const __amdgpu_buffer_rsrc_t rl = row_rsrc(la + row * NSUB, EP * NSUB * 4);
+  const __amdgpu_buffer_rsrc_t rl = row_rsrc(la + 2 * row * NSUB, EP * NSUB * 8);
   const __amdgpu_buffer_rsrc_t rq = row_rsrc(qt + row * NSUB, EP * NSUB * 8);
   if constexpr (NSUB == 2) {
     const u32x4 r = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rr, e * 16, 0, 0));
-    const u32x2 l = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(rl, e * 8, 0, 0));
+    const u32x4 l = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rl, e * 16, 0, 0));
     const u32x4 q = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rq, e * 16, 0, 0));
     t.r[0] = __builtin_bit_cast(double, (u32x2){r.x, r.y});
     t.r[1] = __builtin_bit_cast(double, (u32x2){r.z, r.w});
     t.l[0] = __uint_as_float(l.x);
-    t.l[1] = __uint_as_float(l.y);
+    t.rf[0] = __uint_as_float(l.y);
+    t.l[1] = __uint_as_float(l.z);
+    t.rf[1] = __uint_as_float(l.w);
     t.q[0] = make_float2(__uint_as_float(q.x), __uint_as_float(q.y));
     t.q[1] = make_float2(__uint_as_float(q.z), __uint_as_float(q.w));
   } else {
@@ -435,7 +440,9 @@ __device__ __forceinline__ TableTerm<NSUB> load_term(const double* __restrict__ 
     for (int j = 0; j < NSUB; ++j) {
       const int o = (e * NSUB + j);
       t.r[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rr, o * 8, 0, 0));
-      t.l[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rl, o * 4, 0, 0));
+      const u32x2 l = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(rl, o * 8, 0, 0));
+      t.l[j] = __uint_as_float(l.x);
+      t.rf[j] = __uint_as_float(l.y);
       const u32x2 q = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(rq, o * 8, 0, 0));
       t.q[j] = make_float2(__uint_as_float(q.x), __uint_as_float(q.y));
     }
@@ -447,24 +454,23 @@ __device__ __forceinline__ TableTerm<NSUB> load_term(const double* __restrict__ 
 template <int NSUB>
 struct TableTile {
   TableTerm<NSUB> t[4];
-  int wbits[4];   // bits of w = +-sqrt|rc| / n_sub: the sign of rc (wave-uniform)
-  bool valid[4];  // scatterer inside the unit (wave-uniform)
+  int wbits[4];  // bits of w = +-sqrt|rc| / n_sub: the sign of rc (wave-uniform)
 };
 
 template <int NSUB, int ET>
 __device__ __forceinline__ void fetch_table(const float4* __restrict__ scat, const double* __restrict__ rt,
                                             const float* __restrict__ la, const float2* __restrict__ qt, int s0,
-                                            int s_end, TableTile<NSUB>& tt) {
+                                            int s_end, int s_dead, TableTile<NSUB>& tt) {
   constexpr int EP = ET * 16;
   const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / EP));  // wave-uniform
   const int e = threadIdx.x % EP;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int s = s0 + 4 * g + i;
-    tt.valid[i] = s < s_end;
-    const int sc = tt.valid[i] ? s : s_end - 1;  // in-bounds; masked in gen_fragments_table
-    tt.wbits[i] = __float_as_int(scat[sc].z);
-    tt.t[i] = load_term<NSUB, EP>(rt, la, qt, (size_t)sc * EP, e);
+    const bool valid = s < s_end;
+    // scatterers past the unit's end read the table's dead row (amplitude 2^-inf = 0)
+    tt.wbits[i] = __float_as_int(scat[valid ? s : s_end - 1].z);
+    tt.t[i] = load_term<NSUB, EP>(rt, la, qt, (size_t)(valid ? s : s_dead) * EP, e);
   }
 }
 
@@ -486,8 +492,8 @@ __device__ __forceinline__ void gen_fragments_table(const BlockScalars& bs, cons
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
       const double r = tt.t[i].r[j];
-      const float lj = tt.valid[i] ? tt.t[i].l[j] : -INFINITY;
-      const float rf = (float)r;
+      const float lj = tt.t[i].l[j];
+      const float rf = tt.t[i].rf[j];
       const float ph0 = (float)__builtin_amdgcn_fract(fma(bs.nu0, r, quarter));  // v_fract_f64: x - floor(x)
       const float m0 = __builtin_amdgcn_exp2f(fmaf(-bs.a0, rf, lj));
       cf gv = {m0 * __builtin_amdgcn_cosf(ph0), m0 * __builtin_amdgcn_sinf(ph0)};
@@ -626,7 +632,7 @@ __device__ __forceinline__ void msm_wave(const Problem& p, const float4* __restr
   if constexpr (MODE == MSM_TABLE) {
     TableTile<NSUB> nxt;
     if (sb < se) {
-      fetch_table<NSUB, ET>(scat, rt, la, qt, sb, se, nxt);
+      fetch_table<NSUB, ET>(scat, rt, la, qt, sb, se, p.S, nxt);
       gen_fragments_table<NSUB, FBK, ET>(bs, nxt, frag);
     }
     __syncthreads();
@@ -636,7 +642,7 @@ __device__ __forceinline__ void msm_wave(const Problem& p, const float4* __restr
       if (mfma_first) t.step(frag + buf * L::per_buf, k0);
       if (more) {
         if (!mfma_first) __builtin_amdgcn_s_setprio(1);
-        fetch_table<NSUB, ET>(scat, rt, la, qt, s0 + MST, se, nxt);
+        fetch_table<NSUB, ET>(scat, rt, la, qt, s0 + MST, se, p.S, nxt);
         gen_fragments_table<NSUB, FBK, ET>(bs, nxt, frag + nb * L::per_buf);
         __builtin_amdgcn_s_setprio(0);
       }
@@ -977,11 +983,11 @@ static int msm_plan_et(const Problem& p, int P, const int64_t* off, const fs_tun
   m.kpart_bytes = (size_t)m.units.n * m.nfpad * ep * ep * sizeof(cf);
   m.scat_offset = (m.kpart_bytes + 255) / 256 * 256;
   m.rt_offset = m.scat_offset + ((size_t)off[P] * sizeof(float4) + 255) / 256 * 256;
-  const size_t terms = (size_t)off[P] * ep * p.NS;
-  const size_t table_bytes = terms * (sizeof(double) + sizeof(float) + sizeof(float2));
+  const size_t terms = (size_t)(off[P] + 1) * ep * p.NS;  // + the dead row
+  const size_t table_bytes = terms * (sizeof(double) + 2 * sizeof(float) + sizeof(float2));
   m.table = table_possible(p) && tune.msm_table != 2 && (tune.msm_table == 1 || table_bytes <= MSM_TABLE_AUTO_MAX);
   m.la_offset = m.rt_offset + (m.table ? (terms * sizeof(double) + 255) / 256 * 256 : 0);
-  m.qt_offset = m.la_offset + (m.table ? (terms * sizeof(float) + 255) / 256 * 256 : 0);
+  m.qt_offset = m.la_offset + (m.table ? (terms * 2 * sizeof(float) + 255) / 256 * 256 : 0);
   m.queue_offset = m.qt_offset + (m.table ? (terms * sizeof(float2) + 255) / 256 * 256 : 0);
   m.ws_bytes = m.queue_offset + 256;
   m.lds = FragLds<ET, SH::FBK>::bytes;
@@ -1058,7 +1064,7 @@ template <int NSUB>
 static int launch_msm_et(const Problem& p, const MsmPlan& m, const float4* scat, double* rt, float* la, float2* qt,
                          int* queue, cf* kpart, hipStream_t st) {
   if (m.table && p.S > 0) {
-    const long long n = (long long)p.S * m.ET * 16;
+    const long long n = (long long)(p.S + 1) * m.ET * 16;
     hipLaunchKernelGGL(k_geo_table<NSUB>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, scat, m.ET * 16, rt,
                        la, qt);
   }

@@ -130,7 +130,7 @@ typedef struct fs_tuning {
   int32_t msm_units;     /* single phantom: scatterer chunks (0 = chosen from the CU count) */
   int32_t msm_item_div;  /* batches: largest work item <= 1/msm_item_div of a queue slot's share (0 = 8) */
   int32_t msm_table;     /* geometry table (per scatterer, element, sub-element: FP64 distance, log2 amplitude,
-                            frequency step; 20 B each, in the workspace): 0 = when possible and <= 32 GiB,
+                            frequency step; 24 B each, in the workspace): 0 = when possible and <= 32 GiB,
                             1 = when possible, 2 = never (geometry evaluated per frequency block) */
   int32_t reserved[4];
 } fs_tuning;
