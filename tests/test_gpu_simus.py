@@ -290,3 +290,22 @@ def test_device_plan_matches_host_plan(cuda):
     ph = fs.pfield_precompute(grid, D[0], params, m)
     pd = fs.pfield_precompute(torch.as_tensor(grid, device=cuda), D[0], params, m)
     np.testing.assert_allclose(pd.selected_freqs, ph.selected_freqs, rtol=1e-12)
+
+
+def test_hip_path_never_calls_the_oracle(cuda, monkeypatch):
+    """With every function of the oracle replaced by one that raises, both HIP methods still run
+    (no CPU fallback through the oracle); cf. tests/backend/test_cupy.py:66-95 of the reference."""
+    import types
+
+    import oracle.simus_oracle as OM
+
+    def _boom(*a, **k):
+        raise AssertionError("the oracle was called by the product path")
+
+    for name, obj in list(vars(OM).items()):
+        if isinstance(obj, types.FunctionType):
+            monkeypatch.setattr(OM, name, _boom)
+    params, medium, x, z, rc, D, kw = CASES["l11_speckle_atten"]()
+    for strategy in ("hip", "hip_mfma"):
+        rf, _ = _run(strategy, x, z, rc, D, params, medium, **kw)
+        assert np.isfinite(rf).all() and np.abs(rf).max() > 0

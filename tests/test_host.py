@@ -231,3 +231,24 @@ def test_missing_library_raises_instead_of_falling_back(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(RuntimeError, match="not built"):
         _lib.load()
+
+
+def test_product_code_never_imports_the_oracle():
+    """The package (the product path) has no route to the FP64 oracle: no module of it imports
+    `oracle` (the reference's tests check the same of its CUDA path, tests/backend/test_cupy.py:66-95)."""
+    import ast
+    from pathlib import Path
+
+    import fastsimus_amd
+
+    pkg = Path(fastsimus_amd.__file__).resolve().parent
+    for f in sorted(pkg.rglob("*.py")):
+        tree = ast.parse(f.read_text())
+        for node in ast.walk(tree):
+            if isinstance(node, ast.Import):
+                names = [a.name for a in node.names]
+            elif isinstance(node, ast.ImportFrom):
+                names = [node.module or ""]
+            else:
+                continue
+            assert not any(n == "oracle" or n.startswith("oracle.") for n in names), f"{f.name} imports the oracle"
