@@ -249,34 +249,44 @@ def bench_adjoint(args) -> int:
     with torch.no_grad():
         target = simus_rf(xt, zt, torch.tensor(rc_t, device=dev), 1560.0, D, params, plan, attenuation=att)
 
-    def step():
+    def step(adjoint):
         xg = xt.clone().requires_grad_(True)
         zg = zt.clone().requires_grad_(True)
         rg = torch.tensor(rc, device=dev, requires_grad=True)
         cg = torch.tensor(1540.0, dtype=torch.float64, requires_grad=True)
-        rf = simus_rf(xg, zg, rg, cg, D, params, plan, attenuation=att)
+        rf = simus_rf(xg, zg, rg, cg, D, params, plan, attenuation=att, adjoint=adjoint)
         loss = 0.5 * ((rf - target) ** 2).sum()
         loss.backward()
         return loss
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t_step = (time.perf_counter() - t0) / args.steps
+    def timed(adjoint, steps, warmup):
+        for _ in range(warmup):
+            step(adjoint)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(adjoint)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    t_step = timed(args.adjoint, args.steps, args.warmup)
+    # the other backward kernels on the same step, for comparison (FP32 vector kernels regenerate the
+    # Green's functions per transmit; the matrix-core form once per group of 16 transmits)
+    other = "vector" if args.adjoint != "vector" else "matrix"
+    t_other = timed(other, 2, 1)
     nf = int(plan.selected_freqs.shape[0])
     value = D.shape[0] * x.size * params.n_elements * nf / t_step / 1e9
     out = {
         "metric": "G(scatterer·element·freq)/s, forward + adjoint (dL/d rc, x, z, c)", "value": value, "unit": UNIT,
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_step * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "precision": "split-BF16 (hi+lo, 3 products) MFMA, FP32 accumulate; FP64 r and phase reseed",
         "config": {"workload": "differentiable SIMUS: 256x256 jittered grid (65536 scatterers), 15 plane waves in "
                                "[-7,7] deg, L2 loss vs RF of rc + N(0,0.1) at c = 1560 m/s",
                    "n_scat": int(x.size), "n_elem": params.n_elements, "n_tx": int(D.shape[0]), "n_freq": nf,
-                   "n_sub": plan.n_sub, "method": "hip (FP32) forward + hand-written adjoint"},
+                   "n_sub": plan.n_sub,
+                   "method": "hip_mfma forward (multistatic matrix) + hand-written adjoint", "adjoint": args.adjoint},
+        f"{other}_adjoint_ms_per_step": t_other * 1e3,
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -361,6 +371,8 @@ def main() -> int:
     ap.add_argument("--plain-tx", type=int, default=3, help="transmits timed with the plain FP32 kernel")
     ap.add_argument("--cpu-sample", type=int, default=1000, help="scatterers in the CPU-baseline / FP64 check sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--adjoint", choices=("auto", "matrix", "vector"), default="auto",
+                    help="adjoint15: backward kernels (matrix cores where they apply, or FP32 vector kernels)")
     ap.add_argument("--workload", choices=("planewave75", "phantoms64", "adjoint15", "pfield"), default="planewave75",
                     help="planewave75 (default, the headline metric), phantoms64 (uneven phantom batch, phased "
                          "array, focused beams), adjoint15 (forward + gradient of the differentiable form) or "

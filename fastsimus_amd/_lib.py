@@ -33,6 +33,7 @@ EXPORTS = (
     "fs_rf_adjoint",
     "fs_simus_adjoint_workspace_bytes",
     "fs_simus_adjoint",
+    "fs_set_adjoint_method",
     "fs_pfield_workspace_bytes",
     "fs_pfield",
     "fs_max_distance",
@@ -107,6 +108,8 @@ def load() -> C.CDLL:
     lib.fs_simus_adjoint_workspace_bytes.argtypes = [D]
     lib.fs_simus_adjoint.restype = C.c_int
     lib.fs_simus_adjoint.argtypes = [D, P, P, P, P, P, P, C.c_size_t, P]
+    lib.fs_set_adjoint_method.restype = C.c_int
+    lib.fs_set_adjoint_method.argtypes = [C.c_int]
     lib.fs_pfield_workspace_bytes.restype = C.c_size_t
     lib.fs_pfield_workspace_bytes.argtypes = [D]
     lib.fs_pfield.restype = C.c_int

@@ -26,13 +26,18 @@ from .plan import SimusPlan
 
 
 class _State:
-    def __init__(self, params, plan, delays, apod, attenuation, method):
+    def __init__(self, params, plan, delays, apod, attenuation, method, adjoint):
         self.params = params
         self.plan = plan
         self.delays = delays
         self.apod = apod
         self.attenuation = attenuation
         self.method = method
+        self.adjoint = adjoint
+
+
+# fs_set_adjoint_method codes
+_ADJOINT = {"auto": 0, "matrix": 1, "vector": 2}
 
 
 # From this many transmits on, the multistatic-matrix (MFMA) forward, whose cost does not grow
@@ -84,9 +89,13 @@ class _SimusRF(torch.autograd.Function):
         d_x = torch.empty(S, dtype=torch.float32, device=dev)
         d_z = torch.empty(S, dtype=torch.float32, device=dev)
         d_c = torch.empty(1, dtype=torch.float64, device=dev)
-        ws2 = K._workspace(lib.fs_simus_adjoint_workspace_bytes(prob.dref), dev)
-        s = lib.fs_simus_adjoint(prob.dref, d_spect.data_ptr(), d_rc.data_ptr(), d_x.data_ptr(), d_z.data_ptr(),
-                                 d_c.data_ptr(), ws2.data_ptr(), ws2.numel(), stream)
+        _lib.check(lib.fs_set_adjoint_method(_ADJOINT[st.adjoint]), "fs_set_adjoint_method")
+        try:
+            ws2 = K._workspace(lib.fs_simus_adjoint_workspace_bytes(prob.dref), dev)
+            s = lib.fs_simus_adjoint(prob.dref, d_spect.data_ptr(), d_rc.data_ptr(), d_x.data_ptr(), d_z.data_ptr(),
+                                     d_c.data_ptr(), ws2.data_ptr(), ws2.numel(), stream)
+        finally:
+            lib.fs_set_adjoint_method(0)
         _lib.check(s, "fs_simus_adjoint")
         dtype, cdev = ctx.c_meta
         g_c = d_c.reshape(()).to(dtype=dtype, device=cdev)
@@ -95,13 +104,19 @@ class _SimusRF(torch.autograd.Function):
 
 
 def simus_rf(x: torch.Tensor, z: torch.Tensor, rc: torch.Tensor, speed_of_sound, delays, params: TransducerParams,
-             plan: SimusPlan, *, attenuation: float = 0.0, tx_apodization=None, method: str = "auto") -> torch.Tensor:
+             plan: SimusPlan, *, attenuation: float = 0.0, tx_apodization=None, method: str = "auto",
+             adjoint: str = "auto") -> torch.Tensor:
     """Gated RF ``(n_tx, n_samples, n_elements)``, differentiable in x, z, rc and speed_of_sound.
 
     ``method`` selects the forward: ``"fp32"`` (per-transmit FP32 kernels), ``"mfma"`` (split-BF16
     multistatic matrix, at most 128 elements) or ``"auto"`` (``"mfma"`` from ``MFMA_MIN_TX``
-    transmits on where it applies). The backward is the same hand-written adjoint either way.
+    transmits on where it applies). ``adjoint`` selects the backward kernels: ``"matrix"`` (element
+    contractions as split-BF16 MFMA products, Green's functions generated once for up to 16
+    transmits; at most 128 elements and 2 sub-elements), ``"vector"`` (FP32, Green's functions
+    regenerated per transmit; any size) or ``"auto"`` (``"matrix"`` where it applies).
     """
+    if adjoint not in _ADJOINT:
+        raise ValueError(f"unknown adjoint {adjoint!r}")
     if not isinstance(speed_of_sound, torch.Tensor):
         speed_of_sound = torch.tensor(float(speed_of_sound), dtype=torch.float64)
     dev = K.hip_device(x, z, rc)
@@ -114,4 +129,5 @@ def simus_rf(x: torch.Tensor, z: torch.Tensor, rc: torch.Tensor, speed_of_sound,
         method = "mfma" if (n_tx >= MFMA_MIN_TX and params.n_elements <= 128) else "fp32"
     if method not in ("fp32", "mfma"):
         raise ValueError(f"unknown method {method!r}")
-    return _SimusRF.apply(x, z, rc, speed_of_sound, _State(params, plan, delays, tx_apodization, attenuation, method))
+    return _SimusRF.apply(x, z, rc, speed_of_sound,
+                          _State(params, plan, delays, tx_apodization, attenuation, method, adjoint))

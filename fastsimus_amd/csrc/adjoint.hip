@@ -332,6 +332,332 @@ __global__ void k_ppc(Problem p, int nfp, cf* __restrict__ ppc) {
   }
 }
 
+// ---------------------------------------------------------------- matrix-core form
+// The transmit sums above are contractions over elements with the transmit
+// index free, i.e. products of matrices. Per frequency f, with
+//   R(e, m) = A_t(f, e)        (m = t' < 16),
+//   R(e, m) = conj(g_t(f, e))  (m = 16 + t'),   t = 16 q + t' (transmit group q)
+// and G(s, e) the Green's function summed over sub-elements:
+//   Y = G R                         (16 scatterers x 32, contraction over e)
+//   P_t(s) = PP Y(s, t'),  B_t(s) = Y(s, 16 + t')
+//   L(s, m) = c PP Y(s, m ^ 16)     (so that W = L R^T is the W of k_adj_pos summed over t)
+//   W(s, e) = sum_m L(s, m) R(e, m) = sum_t c PP A_t(e) B_t(s) + c P_t(s) conj(g_t(e))
+// and sum_e G(s, e) W(s, e) = 2 sum_t c P_t(s) B_t(s), so that
+//   dL/drc_s  = 1/2 Re sum_{f,e} G W
+//   dL/dth_s  = rc_s sum_{e,j} a_th Re S0_j + Re(b_th S1_j),
+//   S0_j = sum_f E_j W, S1_j = sum_f f E_j W      (as in k_adj_pos)
+// Y and W^T are split-BF16 MFMA products (hi*hi + hi*lo + lo*hi, FP32
+// accumulation) with the complex parts in the 32-deep contraction
+// ([Re | Im] of 16 complex indices; the A operand negated in its upper half
+// for the real part, halves swapped for the imaginary part). G is generated
+// once per (scatterer, element, frequency) for all transmits of a group
+// instead of once per transmit.
+//
+// Workgroup: 16 scatterers x nw element slices (one wave per 16 elements),
+// AM_FC frequencies. Lane l of wave w holds scatterer s0 + l % 16 and elements
+// 16 w + 4 (l / 16) + c, c < 4: the A-operand layout of G in Y = G R and the
+// result layout of W^T = R L^T, so that G, E_j and W of one term sit in the
+// same lane. Y is summed over the waves through LDS.
+constexpr int AM_FC = 32;    // frequencies per work item: the FP64 reseed interval of the recurrence
+constexpr int AM_TG = 16;    // transmits per group
+constexpr int AM_MAXW = 8;   // element slices per workgroup (E <= 128)
+constexpr int AM_NSUB_MAX = 2;
+
+typedef __bf16 am_bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 am_bf16x2 __attribute__((ext_vector_type(2)));
+typedef float am_floatx4 __attribute__((ext_vector_type(4)));
+typedef float am_floatx2 __attribute__((ext_vector_type(2)));
+typedef unsigned am_u32x4 __attribute__((ext_vector_type(4)));
+
+// (a, b) -> hi = bf16(a) | bf16(b) << 16, lo = the same of the remainders
+__device__ __forceinline__ void am_split(float a, float b, unsigned& hi, unsigned& lo) {
+  const am_bf16x2 h = __builtin_convertvector((am_floatx2){a, b}, am_bf16x2);
+  const unsigned hb = __builtin_bit_cast(unsigned, h);
+  const float ah = __uint_as_float(hb << 16), bh = __uint_as_float(hb & 0xffff0000u);
+  const am_bf16x2 l = __builtin_convertvector((am_floatx2){a - ah, b - bh}, am_bf16x2);
+  hi = hb;
+  lo = __builtin_bit_cast(unsigned, l);
+}
+
+__device__ __forceinline__ am_floatx4 am_mma(am_u32x4 a, am_u32x4 b, am_floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(am_bf16x8, a), __builtin_bit_cast(am_bf16x8, b),
+                                                 c, 0, 0, 0);
+}
+
+__device__ __forceinline__ size_t am_ridx(int k, int nw, int w, int x, int hl, int lane) {
+  return ((((size_t)k * nw + w) * 2 + x) * 2 + hl) * 64 + lane;
+}
+
+// Gc[t][e][k] = conj(g_t(f_k, e)) (0 beyond NF)
+__global__ void k_adjm_conj(Problem p, int nfp, const cf* __restrict__ gspect, cf* __restrict__ Gc) {
+  const long total = (long)p.NT * p.E * nfp;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(i % nfp);
+    const long te = i / nfp;
+    const int e = (int)(te % p.E);
+    const int t = (int)(te / p.E);
+    cf v = {0.f, 0.f};
+    if (k < p.NF) {
+      const cf g = gspect[((size_t)t * p.NF + k) * p.E + e];
+      v = {g.x, -g.y};
+    }
+    Gc[i] = v;
+  }
+}
+
+// Split-BF16 fragments of R for transmit group q, per frequency and element slice w:
+//   R1[k][w][n][hl][lane]: B operand of Y = G R, lane (m = 16 n + l % 16; elements 16 w + 4 (l / 16) + c)
+//   R2[k][w][x][hl][lane]: A operand of W^T = R L^T, lane (element 16 w + l % 16; m = 16 x + 4 (l / 16) + c)
+// each as [Re of the 4 | Im of the 4]; zero for t >= NT and e >= E.
+__global__ void k_adjm_rtab(Problem p, int nfp, int q, int nw, const cf* __restrict__ A, const cf* __restrict__ Gc,
+                            am_u32x4* __restrict__ R1, am_u32x4* __restrict__ R2) {
+  const long total = (long)p.NF * nw * 2 * 64;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int lane = (int)(i & 63);
+    const int x = (int)((i >> 6) & 1);
+    const long kw = i >> 7;
+    const int w = (int)(kw % nw);
+    const int k = (int)(kw / nw);
+    const int g = lane >> 4, col = lane & 15;
+    const cf* src = x == 0 ? A : Gc;
+    float re[4], im[4];
+    unsigned h0, l0, h1, l1, h2, l2, h3, l3;
+    // R1: one transmit, four elements
+    {
+      const int t = AM_TG * q + col;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int e = 16 * w + 4 * g + c;
+        cf v = {0.f, 0.f};
+        if (t < p.NT && e < p.E) v = src[((size_t)t * p.E + e) * nfp + k];
+        re[c] = v.x;
+        im[c] = v.y;
+      }
+      am_split(re[0], re[1], h0, l0);
+      am_split(re[2], re[3], h1, l1);
+      am_split(im[0], im[1], h2, l2);
+      am_split(im[2], im[3], h3, l3);
+      R1[am_ridx(k, nw, w, x, 0, lane)] = (am_u32x4){h0, h1, h2, h3};
+      R1[am_ridx(k, nw, w, x, 1, lane)] = (am_u32x4){l0, l1, l2, l3};
+    }
+    // R2: one element, four transmits
+    {
+      const int e = 16 * w + col;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int t = AM_TG * q + 4 * g + c;
+        cf v = {0.f, 0.f};
+        if (t < p.NT && e < p.E) v = src[((size_t)t * p.E + e) * nfp + k];
+        re[c] = v.x;
+        im[c] = v.y;
+      }
+      am_split(re[0], re[1], h0, l0);
+      am_split(re[2], re[3], h1, l1);
+      am_split(im[0], im[1], h2, l2);
+      am_split(im[2], im[3], h3, l3);
+      R2[am_ridx(k, nw, w, x, 0, lane)] = (am_u32x4){h0, h1, h2, h3};
+      R2[am_ridx(k, nw, w, x, 1, lane)] = (am_u32x4){l0, l1, l2, l3};
+    }
+  }
+}
+
+// One work item: scatterers [16 blockIdx.x, +16), frequencies [AM_FC blockIdx.y, +AM_FC),
+// transmit group q. Writes the item's partial sums (one per scatterer) of dL/drc and, times
+// rc_s, of dL/dx, dL/dz, dL/dc at [(q nch + blockIdx.y) S + s].
+template <int NSUB>
+__global__ __launch_bounds__(64 * AM_MAXW) void k_adjm(Problem p, int q, int nw, const am_u32x4* __restrict__ R1,
+                                                       const am_u32x4* __restrict__ R2, const cf* __restrict__ ppc,
+                                                       float* __restrict__ prc, float* __restrict__ px,
+                                                       float* __restrict__ pz, float* __restrict__ pc) {
+  __shared__ float2 ypart[AM_MAXW * 16 * 32];                // [w][s][m]: Y of each slice
+  __shared__ __attribute__((aligned(16))) unsigned short lfrag[2 * 2 * 64 * 8];  // [x][hl][lane][8]: L^T
+  __shared__ float red[AM_MAXW][4][16];
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4, col = lane & 15;
+  const int nch = gridDim.y;
+  const int k0 = blockIdx.y * AM_FC;
+  const int k1 = min(p.NF, k0 + AM_FC);
+  const int s = blockIdx.x * 16 + col;
+  const bool valid = s < p.S;
+  const float xs = valid ? p.x[s] : 0.f;
+  const float zs = valid ? p.z[s] : 1e-2f;
+  const bool in_field = valid && !out_of_field(p, xs, zs);
+  const double fstart = p.f0 + (double)k0 * p.df;
+
+  // E_j(f) = exp(f r_c (-alpha' + i 2 pi / c)) by FP32 recurrence from an FP64-phase seed at fstart
+  cf E[4][NSUB], Q[4][NSUB];
+  float amp[4][NSUB];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int e = 16 * w + 4 * g + c;
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+      const GeoF geo = geometry_fast(p, xs, zs, e < p.E ? e : p.E - 1, j);
+      amp[c][j] = (in_field && e < p.E) ? geo.amp : 0.f;
+      E[c][j] = green_fast(geo, fstart, 1.0f);
+      Q[c][j] = green_step_fast(geo, p.df);
+    }
+  }
+  cf S0[4][NSUB], S1[4][NSUB];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) S0[c][j] = S1[c][j] = {0.f, 0.f};
+  float racc = 0.f;
+  const am_u32x4 nim = {0u, 0u, 0x80008000u, 0x80008000u};
+  const am_floatx4 zero4 = {0.f, 0.f, 0.f, 0.f};
+
+  for (int k = k0; k < k1; ++k) {
+    am_u32x4 b1[2][2], a2[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int hl = 0; hl < 2; ++hl) {
+        b1[x][hl] = R1[am_ridx(k, nw, w, x, hl, lane)];
+        a2[x][hl] = R2[am_ridx(k, nw, w, x, hl, lane)];
+      }
+    // G of the lane's 4 elements
+    float gr[4], gi[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      gr[c] = amp[c][0] * E[c][0].x;
+      gi[c] = amp[c][0] * E[c][0].y;
+#pragma unroll
+      for (int j = 1; j < NSUB; ++j) {
+        gr[c] = fmaf(amp[c][j], E[c][j].x, gr[c]);
+        gi[c] = fmaf(amp[c][j], E[c][j].y, gi[c]);
+      }
+    }
+    unsigned h0, l0, h1, l1, h2, l2, h3, l3;
+    am_split(gr[0], gr[1], h0, l0);
+    am_split(gr[2], gr[3], h1, l1);
+    am_split(gi[0], gi[1], h2, l2);
+    am_split(gi[2], gi[3], h3, l3);
+    const am_u32x4 a1h = (am_u32x4){h0, h1, h2, h3} ^ nim, a1l = (am_u32x4){l0, l1, l2, l3} ^ nim;
+    const am_u32x4 a2h = {h2, h3, h0, h1}, a2l = {l2, l3, l0, l1};
+    // Y over this slice's 16 elements; result lane (m = 16 x + col, scatterer 4 g + i)
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      am_floatx4 yr = am_mma(a1h, b1[x][0], zero4);
+      am_floatx4 yi = am_mma(a2h, b1[x][0], zero4);
+      yr = am_mma(a1h, b1[x][1], yr);
+      yi = am_mma(a2h, b1[x][1], yi);
+      yr = am_mma(a1l, b1[x][0], yr);
+      yi = am_mma(a2l, b1[x][0], yi);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) ypart[(w * 16 + 4 * g + i) * 32 + 16 * x + col] = make_float2(yr[i], yi[i]);
+    }
+    __syncthreads();
+    // Y summed over the slices in a fixed order; L(s, m ^ 16) = c PP Y(s, m) as split-BF16 B operand of W^T
+    {
+      const cf pp = ppc[2 * k];
+      const float ck = ppc[2 * k + 1].x;
+      const cf cpp = {ck * pp.x, ck * pp.y};
+      for (int ent = threadIdx.x; ent < 512; ent += blockDim.x) {
+        const int sl = ent >> 5, m = ent & 31;
+        float yx = 0.f, yy = 0.f;
+        for (int v = 0; v < nw; ++v) {
+          const float2 t = ypart[(v * 16 + sl) * 32 + m];
+          yx += t.x;
+          yy += t.y;
+        }
+        const cf Lv = cmul(cpp, cf{yx, yy});
+        const int mt = m ^ 16;
+        const int xx = mt >> 4, mm = mt & 15;
+        const int ln = sl + 16 * (mm >> 2), cc = mm & 3;
+        unsigned hb, lb;
+        am_split(Lv.x, Lv.y, hb, lb);
+        unsigned short* fh = lfrag + ((xx * 2 + 0) * 64 + ln) * 8;
+        unsigned short* fl = lfrag + ((xx * 2 + 1) * 64 + ln) * 8;
+        fh[cc] = (unsigned short)(hb & 0xffffu);
+        fh[4 + cc] = (unsigned short)(hb >> 16);
+        fl[cc] = (unsigned short)(lb & 0xffffu);
+        fl[4 + cc] = (unsigned short)(lb >> 16);
+      }
+    }
+    __syncthreads();
+    // W^T for this slice: A = R (upper half negated / halves swapped), B = L^T;
+    // result lane (scatterer col, element 16 w + 4 g + i): the lane's own terms
+    am_floatx4 wr = zero4, wi = zero4;
+    const am_u32x4* lf4 = (const am_u32x4*)lfrag;
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      const am_u32x4 bh = lf4[(x * 2 + 0) * 64 + lane];
+      const am_u32x4 bl = lf4[(x * 2 + 1) * 64 + lane];
+      const am_u32x4 rh = a2[x][0], rl = a2[x][1];
+      const am_u32x4 rnh = rh ^ nim, rnl = rl ^ nim;
+      const am_u32x4 rsh = {rh.z, rh.w, rh.x, rh.y}, rsl = {rl.z, rl.w, rl.x, rl.y};
+      wr = am_mma(rnh, bh, wr);
+      wi = am_mma(rsh, bh, wi);
+      wr = am_mma(rnh, bl, wr);
+      wi = am_mma(rsh, bl, wi);
+      wr = am_mma(rnl, bh, wr);
+      wi = am_mma(rsl, bh, wi);
+    }
+    const float f = (float)(p.f0 + (double)k * p.df);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const cf Wc = {wr[c], wi[c]};
+      racc = fmaf(gr[c], Wc.x, racc);
+      racc = fmaf(-gi[c], Wc.y, racc);
+#pragma unroll
+      for (int j = 0; j < NSUB; ++j) {
+        const cf ew = cmul(E[c][j], Wc);
+        S0[c][j].x += ew.x;
+        S0[c][j].y += ew.y;
+        S1[c][j].x = fmaf(f, ew.x, S1[c][j].x);
+        S1[c][j].y = fmaf(f, ew.y, S1[c][j].y);
+        E[c][j] = cmul(E[c][j], Q[c][j]);
+      }
+    }
+  }
+
+  // derivative coefficients once per item; sums over the lane's terms, its 4 lane groups and the slices
+  float gx = 0.f, gz = 0.f, gcs = 0.f;
+  if (in_field) {
+    const float fc_hz = (float)(p.min_dist > 0 ? 0.5 / (p.min_dist * p.inv_c) : 1.0);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int e = 16 * w + 4 * g + c;
+      if (e >= p.E) continue;
+#pragma unroll
+      for (int j = 0; j < NSUB; ++j) {
+        GeoF geo;
+        Dcoef dc;
+        coeffs_fast(p, xs, zs, e, j, fc_hz, geo, dc);
+        gx += dc.ax * S0[c][j].x + (dc.bx.x * S1[c][j].x - dc.bx.y * S1[c][j].y);
+        gz += dc.az * S0[c][j].x + (dc.bz.x * S1[c][j].x - dc.bz.y * S1[c][j].y);
+        gcs += dc.ac * S0[c][j].x + (dc.bc.x * S1[c][j].x - dc.bc.y * S1[c][j].y);
+      }
+    }
+  }
+  float v[4] = {0.5f * racc, gx, gz, gcs};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    v[i] += __shfl_xor(v[i], 16);
+    v[i] += __shfl_xor(v[i], 32);
+  }
+  if (g == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[w][i][col] = v[i];
+  }
+  __syncthreads();
+  if (threadIdx.x < 16 && valid) {
+    float a[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int u = 0; u < nw; ++u)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] += red[u][i][threadIdx.x];
+    const float rcs = p.rc[s];
+    const bool live = in_field && rcs != 0.f;
+    const size_t o = ((size_t)q * nch + blockIdx.y) * p.S + s;
+    prc[o] = a[0];
+    px[o] = live ? a[1] * rcs : 0.f;
+    pz[o] = live ? a[2] * rcs : 0.f;
+    pc[o] = live ? a[3] * rcs : 0.f;
+  }
+}
+
 // ---------------------------------------------------------------- host side
 void launch_tx_table(const Problem& p, int nfp, cf* A, hipStream_t st);  // simus_fp32.hip
 
@@ -366,7 +692,6 @@ static AdjLayout adj_layout(const Problem& p) {
   return L;
 }
 
-size_t adjoint_workspace(const Problem& p) { return adj_layout(p).total; }
 
 template <int NSUB>
 static void adj_launch(const Problem& p, const AdjLayout& L, char* ws, int t, float* d_rc, float* d_x, float* d_z,
@@ -391,9 +716,8 @@ static void adj_launch(const Problem& p, const AdjLayout& L, char* ws, int t, fl
   hipLaunchKernelGGL(k_sum_parts, g1, dim3(256), 0, st, dz, L.nfb, p.S, t > 0 ? 1 : 0, d_z);
 }
 
-int adjoint_fp32(const Problem& p, const cf* dspect, float* d_rc, float* d_x, float* d_z, double* d_c, char* ws,
-                 hipStream_t st) {
-  if (p.S == 0) return FS_ERR_INVALID;
+static int adjoint_vector(const Problem& p, const cf* dspect, float* d_rc, float* d_x, float* d_z, double* d_c,
+                          char* ws, hipStream_t st) {
   const AdjLayout L = adj_layout(p);
   {
     const long n = (long)p.NT * p.E * L.nfp;
@@ -418,4 +742,98 @@ int adjoint_fp32(const Problem& p, const cf* dspect, float* d_rc, float* d_x, fl
   return hipGetLastError() == hipSuccess ? FS_OK : FS_ERR_LAUNCH;
 }
 
+// Matrix-core form: workspace [A | Gc | ppc | R1 | R2 | partials rc, x, z, c | dc segments].
+struct AdjmLayout {
+  int nfp, nw, ngrp, nch;
+  size_t oA, oGc, oppc, oR1, oR2, orc, ox, oz, oc, oseg, total;
+};
+
+static AdjmLayout adjm_layout(const Problem& p) {
+  AdjmLayout L{};
+  L.nch = (p.NF + AM_FC - 1) / AM_FC;
+  L.nfp = L.nch * AM_FC;
+  L.nw = (p.E + 15) / 16;
+  L.ngrp = (p.NT + AM_TG - 1) / AM_TG;
+  const size_t tab = (size_t)p.NT * p.E * L.nfp * sizeof(cf);
+  const size_t rt = (size_t)p.NF * L.nw * 2 * 2 * 64 * sizeof(am_u32x4);
+  const size_t part = (size_t)L.ngrp * L.nch * p.S * sizeof(float);
+  size_t o = 0;
+  L.oA = o;   o = al(o + tab);
+  L.oGc = o;  o = al(o + tab);
+  L.oppc = o; o = al(o + 2 * (size_t)L.nfp * sizeof(cf));
+  L.oR1 = o;  o = al(o + rt);
+  L.oR2 = o;  o = al(o + rt);
+  L.orc = o;  o = al(o + part);
+  L.ox = o;   o = al(o + part);
+  L.oz = o;   o = al(o + part);
+  L.oc = o;   o = al(o + part);
+  L.oseg = o; o = al(o + DC_SEGMENTS * sizeof(double));
+  L.total = o;
+  return L;
+}
+
+static int adjoint_matrix(const Problem& p, const cf* dspect, float* d_rc, float* d_x, float* d_z, double* d_c,
+                          char* ws, hipStream_t st) {
+  const AdjmLayout L = adjm_layout(p);
+  cf* A = (cf*)(ws + L.oA);
+  cf* Gc = (cf*)(ws + L.oGc);
+  cf* ppc = (cf*)(ws + L.oppc);
+  am_u32x4* R1 = (am_u32x4*)(ws + L.oR1);
+  am_u32x4* R2 = (am_u32x4*)(ws + L.oR2);
+  float* prc = (float*)(ws + L.orc);
+  float* px = (float*)(ws + L.ox);
+  float* pz = (float*)(ws + L.oz);
+  float* pc = (float*)(ws + L.oc);
+  {
+    const long n = (long)p.NT * p.E * L.nfp;
+    const int blocks = (int)std::min<long>((n + 255) / 256, 65536);
+    launch_tx_table(p, L.nfp, A, st);
+    hipLaunchKernelGGL(k_adjm_conj, dim3(blocks), dim3(256), 0, st, p, L.nfp, dspect, Gc);
+    hipLaunchKernelGGL(k_ppc, dim3((L.nfp + 255) / 256), dim3(256), 0, st, p, L.nfp, ppc);
+  }
+  const long nr = (long)p.NF * L.nw * 2 * 64;
+  const int rblocks = (int)std::min<long>((nr + 255) / 256, 65536);
+  const dim3 grid((p.S + 15) / 16, L.nch);
+  for (int q = 0; q < L.ngrp; ++q) {
+    hipLaunchKernelGGL(k_adjm_rtab, dim3(rblocks), dim3(256), 0, st, p, L.nfp, q, L.nw, (const cf*)A,
+                       (const cf*)Gc, R1, R2);
+    if (p.NS == 1)
+      hipLaunchKernelGGL(k_adjm<1>, grid, dim3(64 * L.nw), 0, st, p, q, L.nw, R1, R2, ppc, prc, px, pz, pc);
+    else
+      hipLaunchKernelGGL(k_adjm<2>, grid, dim3(64 * L.nw), 0, st, p, q, L.nw, R1, R2, ppc, prc, px, pz, pc);
+  }
+  const int nb = L.ngrp * L.nch;
+  const dim3 g1((p.S + 255) / 256);
+  hipLaunchKernelGGL(k_sum_parts, g1, dim3(256), 0, st, prc, nb, p.S, 0, d_rc);
+  hipLaunchKernelGGL(k_sum_parts, g1, dim3(256), 0, st, px, nb, p.S, 0, d_x);
+  hipLaunchKernelGGL(k_sum_parts, g1, dim3(256), 0, st, pz, nb, p.S, 0, d_z);
+  hipLaunchKernelGGL(k_sum_dc_segments, dim3(DC_SEGMENTS), dim3(256), 0, st, (const float*)pc, (long)nb * p.S,
+                     (double*)(ws + L.oseg));
+  hipLaunchKernelGGL(k_sum_dc, dim3(1), dim3(256), 0, st, (const double*)(ws + L.oseg), d_c);
+  return hipGetLastError() == hipSuccess ? FS_OK : FS_ERR_LAUNCH;
+}
+
+// 0: matrix cores where they apply (n_elem <= 128, n_sub <= 2), else vector; 1: matrix cores only; 2: vector only.
+static int g_adjoint_method = 0;
+
+static bool matrix_applies(const Problem& p) { return p.E <= 16 * AM_MAXW && p.NS <= AM_NSUB_MAX; }
+
+static bool use_matrix(const Problem& p) { return g_adjoint_method != 2 && matrix_applies(p); }
+
+size_t adjoint_workspace(const Problem& p) { return use_matrix(p) ? adjm_layout(p).total : adj_layout(p).total; }
+
+int adjoint_fp32(const Problem& p, const cf* dspect, float* d_rc, float* d_x, float* d_z, double* d_c, char* ws,
+                 hipStream_t st) {
+  if (p.S == 0) return FS_ERR_INVALID;
+  if (g_adjoint_method == 1 && !matrix_applies(p)) return FS_ERR_UNSUPPORTED;
+  if (use_matrix(p)) return adjoint_matrix(p, dspect, d_rc, d_x, d_z, d_c, ws, st);
+  return adjoint_vector(p, dspect, d_rc, d_x, d_z, d_c, ws, st);
+}
+
 }  // namespace fs
+
+extern "C" int fs_set_adjoint_method(int method) {
+  if (method < 0 || method > 2) return FS_ERR_INVALID;
+  fs::g_adjoint_method = method;
+  return FS_OK;
+}

@@ -17,7 +17,8 @@
  *   fs_spectrum_to_rf     <- _irfft_and_threshold, src/fast_simus/simus.py:336-367
  *   fs_simus_adjoint      <- no kernel in the reference; differentiates the
  *                            algebra of _simus_freq_outer_scan
- *                            (src/fast_simus/_simus_strategies.py:152-210)
+ *                            (src/fast_simus/_simus_strategies.py:152-210);
+ *                            fs_set_adjoint_method selects its kernels
  *   fs_rf_adjoint         <- reverse of _irfft_and_threshold (simus.py:336-367)
  *   fs_pfield             <- pfield_metal / _freq_outer_python
  *                            (src/fast_simus/kernels/metal_pfield.py:89-219,
@@ -163,6 +164,13 @@ int fs_rf_adjoint(const float* rf_ungated, const float* d_rf, int n_tx, int n_fr
 size_t fs_simus_adjoint_workspace_bytes(const fs_simus_desc* d);
 int fs_simus_adjoint(const fs_simus_desc* d, const float* d_spect, float* d_rc, float* d_x, float* d_z, double* d_c,
                      void* workspace, size_t workspace_bytes, void* stream);
+/* Kernels of fs_simus_adjoint (and of its workspace query), process-wide: 0 = matrix cores where they
+ * apply (n_elem <= 128, n_sub <= 2: the element contractions as split-BF16 MFMA products, Green's
+ * functions generated once per group of 16 transmits), else FP32 vector kernels; 1 = matrix cores
+ * only (FS_ERR_UNSUPPORTED where they do not apply); 2 = vector kernels only (Green's functions
+ * regenerated per transmit). Not thread-safe; set it before the workspace query. No counterpart in
+ * the reference. */
+int fs_set_adjoint_method(int method);
 
 /* RMS pressure field of one transmit on n_scat points (scat_x/scat_z reused as the grid; rc unused):
  *   out[p] = sqrt(correction * sum_k |pulse_k probe_k sum_{e,j} g A|^2), 0 where out of field. */

@@ -88,8 +88,9 @@ def test_torch_oracle_gradient_matches_finite_differences():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("method", ["fp32", "mfma"])
-def test_hip_adjoint_matches_fp64_gradient(method, cuda):
+@pytest.mark.parametrize("method,adjoint", [("fp32", "vector"), ("fp32", "matrix"), ("mfma", "matrix"),
+                                            ("mfma", "vector")])
+def test_hip_adjoint_matches_fp64_gradient(method, adjoint, cuda):
     from fastsimus_amd.autograd import simus_rf
 
     x, z, rc, rc_t, D = _scene(10)
@@ -103,7 +104,7 @@ def test_hip_adjoint_matches_fp64_gradient(method, cuda):
     zt = torch.tensor(z, dtype=torch.float32, device=cuda, requires_grad=True)
     rt = torch.tensor(rc, dtype=torch.float32, device=cuda, requires_grad=True)
     ct = torch.tensor(1540.0, dtype=torch.float64, requires_grad=True)
-    rf = simus_rf(xt, zt, rt, ct, D, PARAMS, plan, attenuation=ATT, method=method)
+    rf = simus_rf(xt, zt, rt, ct, D, PARAMS, plan, attenuation=ATT, method=method, adjoint=adjoint)
     tgt = target.to(torch.float32).to(cuda)
     loss = 0.5 * ((rf - tgt) ** 2).sum()
     loss.backward()
@@ -169,3 +170,88 @@ def test_hip_adjoint_matches_finite_differences(cuda):
     fd = (run(x, z, rc, 1540.0 + h) - run(x, z, rc, 1540.0 - h)) / (2 * h)
     print("c", fd, gc, flush=True)
     assert fd == pytest.approx(gc, rel=2e-2)
+
+
+def _spectral_adjoint(prob, g, adjoint):
+    """fs_simus_adjoint of the cotangent g (n_tx, n_freq, E, 2) with the given kernels."""
+    from fastsimus_amd import _lib
+    from fastsimus_amd import kernels as K
+
+    lib = _lib.load()
+    code = {"matrix": 1, "vector": 2}[adjoint]
+    S = prob.x.numel()
+    # the C ABI reads n_tx * n_freq * n_elem interleaved complex values: check the buffer on the host
+    assert tuple(g.shape) == (prob.n_tx, prob.n_freq, prob.params.n_elements, 2)
+    assert g.dtype == torch.float32 and g.is_contiguous() and g.device == prob.device
+    out = [torch.empty(S, dtype=torch.float32, device=prob.device) for _ in range(3)]
+    d_c = torch.empty(1, dtype=torch.float64, device=prob.device)
+    _lib.check(lib.fs_set_adjoint_method(code), "fs_set_adjoint_method")
+    try:
+        ws = K._workspace(lib.fs_simus_adjoint_workspace_bytes(prob.dref), prob.device)
+        st = lib.fs_simus_adjoint(prob.dref, g.data_ptr(), *(t.data_ptr() for t in out), d_c.data_ptr(), ws.data_ptr(),
+                                  ws.numel(), K._stream(prob.device))
+    finally:
+        lib.fs_set_adjoint_method(0)
+    _lib.check(st, "fs_simus_adjoint")
+    torch.cuda.synchronize()
+    return [t.cpu().numpy().astype(np.float64) for t in out] + [float(d_c.cpu())]
+
+
+def _adjoint_case(name):
+    """(params, medium, x, z, rc, delays, element_splitting): a few hundred scatterers, several transmits."""
+    if name == "l11_soft_t20":
+        p = fs.L11_5v()
+        x, z, rc = speckle(71, 500, zlim=(0.004, 0.04))
+        z[:12] = -np.abs(z[:12])                      # out of field
+        rc[12:20] = 0.0                               # zero reflectivity: d rc only
+        pos, _, _ = fs.element_positions(p.n_elements, p.pitch, p.radius)
+        D = np.stack([fs.plane_wave(pos, np.deg2rad(a), speed_of_sound=1540.0) for a in np.linspace(-10, 10, 20)])
+        return p, fs.MediumParams(attenuation=0.5), x, z, rc, D, 2
+    if name == "rigid_e100_t3":
+        p = fs.TransducerParams(freq_center=5e6, pitch=3.0e-4, width=2.5e-4, n_elements=100, bandwidth=0.7,
+                                baffle="rigid")
+        x, z, rc = speckle(72, 300, xlim=(-0.015, 0.015), zlim=(0.003, 0.03))
+        pos, _, _ = fs.element_positions(p.n_elements, p.pitch, p.radius)
+        D = np.stack([fs.focused(pos, np.array([xf, 0.02]), speed_of_sound=1540.0) for xf in (-0.004, 0.0, 0.005)])
+        return p, fs.MediumParams(attenuation=0.3), x, z, rc, D, 1
+    if name == "convex_impedance_t17":
+        p = fs.C5_2v().model_copy(update={"baffle": 1.5})
+        x, z, rc = speckle(73, 300, xlim=(-0.03, 0.03), zlim=(0.005, 0.06))
+        pos, _, _ = fs.element_positions(p.n_elements, p.pitch, p.radius)
+        D = np.stack([fs.plane_wave(pos, np.deg2rad(a), speed_of_sound=1540.0) for a in np.linspace(-8, 8, 17)])
+        return p, fs.MediumParams(attenuation=0.4), x, z, rc, D, 2
+    raise KeyError(name)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["l11_soft_t20", "rigid_e100_t3", "convex_impedance_t17"])
+def test_matrix_adjoint_matches_vector_adjoint(case, cuda):
+    """The matrix-core adjoint against the FP32 vector adjoint on the same cotangent (more than one group
+    of 16 transmits, ragged element count, out-of-field and zero-rc scatterers, every baffle kind, a convex
+    array), and both against the forward through Euler's identity for the degree-1 homogeneous map
+    rc -> X: sum_s rc_s dL/drc_s = L for L = Re sum conj(g) X."""
+    from fastsimus_amd import kernels as K
+
+    p, med, x, z, rc, D, nsub = _adjoint_case(case)
+    sc = np.stack([x, z], -1)
+    plan = fs.simus_precompute(sc, rc, D, p, med, element_splitting=nsub)
+    prob = K.Problem(sc, rc, D, None, p, med, n_sub=plan.n_sub, seg_length=plan.seg_length, freqs=plan.selected_freqs,
+                     pulse=plan.pulse_spectrum, probe=plan.probe_spectrum, correction=plan.correction_factor,
+                     full_frequency_directivity=False, device=cuda)
+    X = torch.view_as_real(K.run_spectrum(prob, "fp32")).double().cpu().numpy()   # (n_tx, n_freq, E, 2)
+    rng = np.random.default_rng(11)
+    g = rng.standard_normal(X.shape).astype(np.float32)
+    L = float((g.astype(np.float64) * X).sum())   # Re sum conj(g) X with g, X as (re, im) pairs
+    gt = torch.as_tensor(g, device=cuda)
+    mat = _spectral_adjoint(prob, gt, "matrix")
+    vec = _spectral_adjoint(prob, gt, "vector")
+    for name, a, b in zip(("rc", "x", "z"), mat[:3], vec[:3]):
+        err = rel_l2(a, b)
+        print(case, name, "matrix vs vector rel L2", err, flush=True)
+        assert err < 1e-3, name
+    assert mat[3] == pytest.approx(vec[3], rel=1e-3)
+    live = (z >= 0)
+    assert np.all(mat[0][~live] == 0) and np.all(mat[1][~live] == 0) and np.all(mat[2][~live] == 0)
+    assert np.all(mat[1][rc == 0] == 0) and np.all(mat[2][rc == 0] == 0)
+    for d_rc in (mat[0], vec[0]):
+        assert float((rc * d_rc).sum()) == pytest.approx(L, rel=1e-3)
