@@ -362,6 +362,7 @@ constexpr int AM_FC = 32;    // frequencies per work item: the FP64 reseed inter
 constexpr int AM_TG = 16;    // transmits per group
 constexpr int AM_MAXW = 8;   // element slices per workgroup (E <= 128)
 constexpr int AM_NSUB_MAX = 2;
+constexpr int AM_YROW = 16;  // LDS row of Y: 16 scatterers
 
 typedef __bf16 am_bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 am_bf16x2 __attribute__((ext_vector_type(2)));
@@ -408,10 +409,10 @@ __global__ void k_adjm_conj(Problem p, int nfp, const cf* __restrict__ gspect, c
 // Split-BF16 fragments of R for transmit group q, per frequency and element slice w:
 //   R1[k][w][n][hl][lane]: B operand of Y = G R, lane (m = 16 n + l % 16; elements 16 w + 4 (l / 16) + c)
 //   R2[k][w][x][hl][lane]: A operand of W^T = R L^T, lane (element 16 w + l % 16; m = 16 x + 4 (l / 16) + c)
-// each as [Re of the 4 | Im of the 4]; zero for t >= NT and e >= E.
+// each as [Re of the 4 | Im of the 4]; zero for t >= NT, e >= E and k >= NF (k < nfp).
 __global__ void k_adjm_rtab(Problem p, int nfp, int q, int nw, const cf* __restrict__ A, const cf* __restrict__ Gc,
                             am_u32x4* __restrict__ R1, am_u32x4* __restrict__ R2) {
-  const long total = (long)p.NF * nw * 2 * 64;
+  const long total = (long)nfp * nw * 2 * 64;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
     const int lane = (int)(i & 63);
     const int x = (int)((i >> 6) & 1);
@@ -429,7 +430,7 @@ __global__ void k_adjm_rtab(Problem p, int nfp, int q, int nw, const cf* __restr
       for (int c = 0; c < 4; ++c) {
         const int e = 16 * w + 4 * g + c;
         cf v = {0.f, 0.f};
-        if (t < p.NT && e < p.E) v = src[((size_t)t * p.E + e) * nfp + k];
+        if (t < p.NT && e < p.E && k < p.NF) v = src[((size_t)t * p.E + e) * nfp + k];
         re[c] = v.x;
         im[c] = v.y;
       }
@@ -447,7 +448,7 @@ __global__ void k_adjm_rtab(Problem p, int nfp, int q, int nw, const cf* __restr
       for (int c = 0; c < 4; ++c) {
         const int t = AM_TG * q + 4 * g + c;
         cf v = {0.f, 0.f};
-        if (t < p.NT && e < p.E) v = src[((size_t)t * p.E + e) * nfp + k];
+        if (t < p.NT && e < p.E && k < p.NF) v = src[((size_t)t * p.E + e) * nfp + k];
         re[c] = v.x;
         im[c] = v.y;
       }
@@ -461,23 +462,136 @@ __global__ void k_adjm_rtab(Problem p, int nfp, int q, int nw, const cf* __restr
   }
 }
 
-// One work item: scatterers [16 blockIdx.x, +16), frequencies [AM_FC blockIdx.y, +AM_FC),
-// transmit group q. Writes the item's partial sums (one per scatterer) of dL/drc and, times
-// rc_s, of dL/dx, dL/dz, dL/dc at [(q nch + blockIdx.y) S + s].
+// Operands of Y = G R for one frequency: G split as [Re | -Im] (real part) and [Im | Re]
+// (imaginary part), hi and lo, and G itself for the rc gradient.
+struct AmOps {
+  am_u32x4 a1h, a1l, a2h, a2l;
+  float gr[4], gi[4];
+};
+
 template <int NSUB>
-__global__ __launch_bounds__(64 * AM_MAXW) void k_adjm(Problem p, int q, int nw, const am_u32x4* __restrict__ R1,
-                                                       const am_u32x4* __restrict__ R2, const cf* __restrict__ ppc,
-                                                       float* __restrict__ prc, float* __restrict__ px,
-                                                       float* __restrict__ pz, float* __restrict__ pc) {
-  __shared__ float2 ypart[AM_MAXW * 16 * 32];                // [w][s][m]: Y of each slice
-  __shared__ __attribute__((aligned(16))) unsigned short lfrag[2 * 2 * 64 * 8];  // [x][hl][lane][8]: L^T
-  __shared__ float red[AM_MAXW][4][16];
+__device__ __forceinline__ void am_gen(const float (&amp)[4][NSUB], const cf (&E)[4][NSUB], AmOps& o) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    o.gr[c] = amp[c][0] * E[c][0].x;
+    o.gi[c] = amp[c][0] * E[c][0].y;
+#pragma unroll
+    for (int j = 1; j < NSUB; ++j) {
+      o.gr[c] = fmaf(amp[c][j], E[c][j].x, o.gr[c]);
+      o.gi[c] = fmaf(amp[c][j], E[c][j].y, o.gi[c]);
+    }
+  }
+  const am_u32x4 nim = {0u, 0u, 0x80008000u, 0x80008000u};
+  unsigned h0, l0, h1, l1, h2, l2, h3, l3;
+  am_split(o.gr[0], o.gr[1], h0, l0);
+  am_split(o.gr[2], o.gr[3], h1, l1);
+  am_split(o.gi[0], o.gi[1], h2, l2);
+  am_split(o.gi[2], o.gi[3], h3, l3);
+  o.a1h = (am_u32x4){h0, h1, h2, h3} ^ nim;
+  o.a1l = (am_u32x4){l0, l1, l2, l3} ^ nim;
+  o.a2h = (am_u32x4){h2, h3, h0, h1};
+  o.a2l = (am_u32x4){l2, l3, l0, l1};
+}
+
+// Y over one slice's 16 elements; result lane (m = 16 x + col, scatterers 4 g .. 4 g + 3)
+__device__ __forceinline__ void am_mfma1(const AmOps& o, const am_u32x4 (&b1)[2][2], am_floatx4 (&yr)[2],
+                                         am_floatx4 (&yi)[2]) {
+  const am_floatx4 zero4 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
+    yr[x] = am_mma(o.a1h, b1[x][0], zero4);
+    yi[x] = am_mma(o.a2h, b1[x][0], zero4);
+  }
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
+    yr[x] = am_mma(o.a1h, b1[x][1], yr[x]);
+    yi[x] = am_mma(o.a2h, b1[x][1], yi[x]);
+  }
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
+    yr[x] = am_mma(o.a1l, b1[x][0], yr[x]);
+    yi[x] = am_mma(o.a2l, b1[x][0], yi[x]);
+  }
+}
+
+// Y of every slice summed in a fixed order; L(s, m ^ 16) = c PP Y(s, m) written as the two
+// split-BF16 B operands of W^T = R L^T: [Re L | -Im L] (real part) and [Im L | Re L] (imaginary part).
+// LDS: ypr / ypi [w][m][s] (rows of AM_YROW floats); lfrag [x][part][hl][lane][8].
+template <int NW>
+__device__ __forceinline__ void am_reduce(const float* ypr, const float* ypi, unsigned short* lfrag, cf cpp) {
+#pragma unroll
+  for (int r = 0; r < 512 / (64 * NW); ++r) {
+    const int ent = threadIdx.x + r * 64 * NW;
+    const int m = ent >> 4, sl = ent & 15;
+    float tr[NW], ti[NW];
+#pragma unroll
+    for (int v = 0; v < NW; ++v) {
+      tr[v] = ypr[(v * 32 + m) * AM_YROW + sl];
+      ti[v] = ypi[(v * 32 + m) * AM_YROW + sl];
+    }
+    float yx = 0.f, yy = 0.f;
+#pragma unroll
+    for (int v = 0; v < NW; ++v) {
+      yx += tr[v];
+      yy += ti[v];
+    }
+    const cf Lv = cmul(cpp, cf{yx, yy});
+    const int mt = m ^ 16;
+    const int xx = mt >> 4, mm = mt & 15;
+    const int ln = sl + 16 * (mm >> 2), cc = mm & 3;
+    unsigned hb, lb;
+    am_split(Lv.x, Lv.y, hb, lb);
+    const unsigned short rh = (unsigned short)(hb & 0xffffu), ih = (unsigned short)(hb >> 16);
+    const unsigned short rl = (unsigned short)(lb & 0xffffu), il = (unsigned short)(lb >> 16);
+    unsigned short* f = lfrag + (xx * 4 * 64 + ln) * 8;   // [x][part 0, hl 0]
+    f[cc] = rh;                                           // part 0 (real), hi
+    f[4 + cc] = (unsigned short)(ih ^ 0x8000u);
+    f[64 * 8 + cc] = rl;                                  // part 0, lo
+    f[64 * 8 + 4 + cc] = (unsigned short)(il ^ 0x8000u);
+    f[2 * 64 * 8 + cc] = ih;                              // part 1 (imaginary), hi
+    f[2 * 64 * 8 + 4 + cc] = rh;
+    f[3 * 64 * 8 + cc] = il;                              // part 1, lo
+    f[3 * 64 * 8 + 4 + cc] = rl;
+  }
+}
+
+// G(f) of the lane's 4 elements (kept for the rc gradient two stages later)
+struct AmG {
+  float gr[4], gi[4];
+};
+
+// One work item: scatterers [16 blockIdx.x, +16), the AM_FC frequencies [AM_FC blockIdx.y, +AM_FC)
+// (the tables are zero past the band, so a padded frequency adds nothing), transmit group q, NW
+// element slices (NW waves, NW * 16 >= E). Writes the item's partial sums (one per scatterer) of
+// dL/drc and, times rc_s, of dL/dx, dL/dz, dL/dc at [(q nch + blockIdx.y) S + s].
+//
+// Three-stage software pipeline over the frequencies, one workgroup barrier per stage step:
+//   A(i): G(f_i) and this slice's Y(f_i) = G R on the matrix cores, stored to LDS buffer i % 2;
+//   B(i - 1): Y(f_{i-1}) summed over the slices into L(f_{i-1}) (LDS buffer (i - 1) % 2);
+//   C(i - 2): W^T(f_{i-2}) = R L^T on the matrix cores and the lane's terms of f_{i-2}.
+// Every LDS buffer is written one step after its last reader's step, so one barrier per step
+// orders all of them, and each wave's vector work overlaps its own matrix products and the LDS
+// latencies of the other stages. E, G of three consecutive frequencies rotate through three
+// register sets (the step sequence is unrolled by three), so nothing is copied.
+template <int NSUB, int NW>
+__global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, const am_u32x4* __restrict__ R1,
+                                                  const am_u32x4* __restrict__ R2, const cf* __restrict__ ppc,
+                                                  float* __restrict__ prc, float* __restrict__ px,
+                                                  float* __restrict__ pz, float* __restrict__ pc) {
+  static_assert(AM_FC % 3 == 2, "the steady steps come in groups of three");
+  constexpr int nw = NW;
+  constexpr int YB = NW * 32 * AM_YROW;       // floats per Y buffer
+  constexpr int LB = 2 * 2 * 2 * 64 * 8;      // shorts per L buffer
+  __shared__ __attribute__((aligned(16))) float ypr[2 * YB];   // [buf][w][m][s]: Re Y of each slice
+  __shared__ __attribute__((aligned(16))) float ypi[2 * YB];   // Im Y
+  __shared__ __attribute__((aligned(16))) unsigned short lfrag[2 * LB];  // [buf][x][part][hl][lane][8]
+  __shared__ float red[NW][4][16];
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4, col = lane & 15;
   const int nch = gridDim.y;
   const int k0 = blockIdx.y * AM_FC;
-  const int k1 = min(p.NF, k0 + AM_FC);
+  const int klast = nch * AM_FC - 1;          // last padded frequency (tables hold nch * AM_FC)
   const int s = blockIdx.x * 16 + col;
   const bool valid = s < p.S;
   const float xs = valid ? p.x[s] : 0.f;
@@ -486,7 +600,7 @@ __global__ __launch_bounds__(64 * AM_MAXW) void k_adjm(Problem p, int q, int nw,
   const double fstart = p.f0 + (double)k0 * p.df;
 
   // E_j(f) = exp(f r_c (-alpha' + i 2 pi / c)) by FP32 recurrence from an FP64-phase seed at fstart
-  cf E[4][NSUB], Q[4][NSUB];
+  cf EA[4][NSUB], EB[4][NSUB], EC[4][NSUB], Q[4][NSUB];
   float amp[4][NSUB];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
@@ -495,123 +609,131 @@ __global__ __launch_bounds__(64 * AM_MAXW) void k_adjm(Problem p, int q, int nw,
     for (int j = 0; j < NSUB; ++j) {
       const GeoF geo = geometry_fast(p, xs, zs, e < p.E ? e : p.E - 1, j);
       amp[c][j] = (in_field && e < p.E) ? geo.amp : 0.f;
-      E[c][j] = green_fast(geo, fstart, 1.0f);
+      EC[c][j] = green_fast(geo, fstart, 1.0f);
       Q[c][j] = green_step_fast(geo, p.df);
+      EA[c][j] = EB[c][j] = {0.f, 0.f};
     }
   }
+  AmG GA, GB, GC;
   cf S0[4][NSUB], S1[4][NSUB];
 #pragma unroll
   for (int c = 0; c < 4; ++c)
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) S0[c][j] = S1[c][j] = {0.f, 0.f};
   float racc = 0.f;
-  const am_u32x4 nim = {0u, 0u, 0x80008000u, 0x80008000u};
-  const am_floatx4 zero4 = {0.f, 0.f, 0.f, 0.f};
+  am_u32x4 b1[2][2], a2[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int hl = 0; hl < 2; ++hl) {
+      b1[x][hl] = R1[am_ridx(k0, nw, w, x, hl, lane)];
+      a2[x][hl] = R2[am_ridx(k0, nw, w, x, hl, lane)];
+    }
 
-  for (int k = k0; k < k1; ++k) {
-    am_u32x4 b1[2][2], a2[2][2];
+  // stage A(i): operands and Y of f_i from E(f_i); stored to Y buffer i % 2
+  auto stage_a = [&](int i, const cf (&Ei)[4][NSUB], AmG& Gi) {
+    AmOps o;
+    am_gen<NSUB>(amp, Ei, o);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      Gi.gr[c] = o.gr[c];
+      Gi.gi[c] = o.gi[c];
+    }
+    am_floatx4 yr[2], yi[2];
+    am_mfma1(o, b1, yr, yi);
+    const int kn = min(k0 + i + 1, klast);
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
-      for (int hl = 0; hl < 2; ++hl) {
-        b1[x][hl] = R1[am_ridx(k, nw, w, x, hl, lane)];
-        a2[x][hl] = R2[am_ridx(k, nw, w, x, hl, lane)];
-      }
-    // G of the lane's 4 elements
-    float gr[4], gi[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      gr[c] = amp[c][0] * E[c][0].x;
-      gi[c] = amp[c][0] * E[c][0].y;
-#pragma unroll
-      for (int j = 1; j < NSUB; ++j) {
-        gr[c] = fmaf(amp[c][j], E[c][j].x, gr[c]);
-        gi[c] = fmaf(amp[c][j], E[c][j].y, gi[c]);
-      }
-    }
-    unsigned h0, l0, h1, l1, h2, l2, h3, l3;
-    am_split(gr[0], gr[1], h0, l0);
-    am_split(gr[2], gr[3], h1, l1);
-    am_split(gi[0], gi[1], h2, l2);
-    am_split(gi[2], gi[3], h3, l3);
-    const am_u32x4 a1h = (am_u32x4){h0, h1, h2, h3} ^ nim, a1l = (am_u32x4){l0, l1, l2, l3} ^ nim;
-    const am_u32x4 a2h = {h2, h3, h0, h1}, a2l = {l2, l3, l0, l1};
-    // Y over this slice's 16 elements; result lane (m = 16 x + col, scatterer 4 g + i)
+      for (int hl = 0; hl < 2; ++hl) b1[x][hl] = R1[am_ridx(kn, nw, w, x, hl, lane)];
+    float* yb = ypr + (i & 1) * YB;
+    float* ib = ypi + (i & 1) * YB;
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
-      am_floatx4 yr = am_mma(a1h, b1[x][0], zero4);
-      am_floatx4 yi = am_mma(a2h, b1[x][0], zero4);
-      yr = am_mma(a1h, b1[x][1], yr);
-      yi = am_mma(a2h, b1[x][1], yi);
-      yr = am_mma(a1l, b1[x][0], yr);
-      yi = am_mma(a2l, b1[x][0], yi);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) ypart[(w * 16 + 4 * g + i) * 32 + 16 * x + col] = make_float2(yr[i], yi[i]);
+      *(am_floatx4*)(yb + (w * 32 + 16 * x + col) * AM_YROW + 4 * g) = yr[x];
+      *(am_floatx4*)(ib + (w * 32 + 16 * x + col) * AM_YROW + 4 * g) = yi[x];
     }
-    __syncthreads();
-    // Y summed over the slices in a fixed order; L(s, m ^ 16) = c PP Y(s, m) as split-BF16 B operand of W^T
-    {
-      const cf pp = ppc[2 * k];
-      const float ck = ppc[2 * k + 1].x;
-      const cf cpp = {ck * pp.x, ck * pp.y};
-      for (int ent = threadIdx.x; ent < 512; ent += blockDim.x) {
-        const int sl = ent >> 5, m = ent & 31;
-        float yx = 0.f, yy = 0.f;
-        for (int v = 0; v < nw; ++v) {
-          const float2 t = ypart[(v * 16 + sl) * 32 + m];
-          yx += t.x;
-          yy += t.y;
-        }
-        const cf Lv = cmul(cpp, cf{yx, yy});
-        const int mt = m ^ 16;
-        const int xx = mt >> 4, mm = mt & 15;
-        const int ln = sl + 16 * (mm >> 2), cc = mm & 3;
-        unsigned hb, lb;
-        am_split(Lv.x, Lv.y, hb, lb);
-        unsigned short* fh = lfrag + ((xx * 2 + 0) * 64 + ln) * 8;
-        unsigned short* fl = lfrag + ((xx * 2 + 1) * 64 + ln) * 8;
-        fh[cc] = (unsigned short)(hb & 0xffffu);
-        fh[4 + cc] = (unsigned short)(hb >> 16);
-        fl[cc] = (unsigned short)(lb & 0xffffu);
-        fl[4 + cc] = (unsigned short)(lb >> 16);
-      }
-    }
-    __syncthreads();
-    // W^T for this slice: A = R (upper half negated / halves swapped), B = L^T;
-    // result lane (scatterer col, element 16 w + 4 g + i): the lane's own terms
-    am_floatx4 wr = zero4, wi = zero4;
-    const am_u32x4* lf4 = (const am_u32x4*)lfrag;
+  };
+  // stage B(i): L of f_i from Y buffer i % 2 into L buffer i % 2
+  auto stage_b = [&](int i) {
+    const cf pp = ppc[2 * (k0 + i)];
+    const float ck = ppc[2 * (k0 + i) + 1].x;
+    am_reduce<NW>(ypr + (i & 1) * YB, ypi + (i & 1) * YB, lfrag + (i & 1) * LB, cf{ck * pp.x, ck * pp.y});
+  };
+  // stage C(i): W^T of f_i (A = R as stored, B = the two variants of L^T; result lane = the lane's
+  // own terms) and the terms of f_i
+  auto stage_c = [&](int i, const cf (&Ei)[4][NSUB], const AmG& Gi) {
+    const am_u32x4* lf4 = (const am_u32x4*)(lfrag + (i & 1) * LB);
+    am_floatx4 wr = {0.f, 0.f, 0.f, 0.f}, wi = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
-      const am_u32x4 bh = lf4[(x * 2 + 0) * 64 + lane];
-      const am_u32x4 bl = lf4[(x * 2 + 1) * 64 + lane];
-      const am_u32x4 rh = a2[x][0], rl = a2[x][1];
-      const am_u32x4 rnh = rh ^ nim, rnl = rl ^ nim;
-      const am_u32x4 rsh = {rh.z, rh.w, rh.x, rh.y}, rsl = {rl.z, rl.w, rl.x, rl.y};
-      wr = am_mma(rnh, bh, wr);
-      wi = am_mma(rsh, bh, wi);
-      wr = am_mma(rnh, bl, wr);
-      wi = am_mma(rsh, bl, wi);
-      wr = am_mma(rnl, bh, wr);
-      wi = am_mma(rsl, bh, wi);
+      const am_u32x4 brh = lf4[(x * 4 + 0) * 64 + lane];
+      const am_u32x4 brl = lf4[(x * 4 + 1) * 64 + lane];
+      const am_u32x4 bih = lf4[(x * 4 + 2) * 64 + lane];
+      const am_u32x4 bil = lf4[(x * 4 + 3) * 64 + lane];
+      wr = am_mma(a2[x][0], brh, wr);
+      wi = am_mma(a2[x][0], bih, wi);
+      wr = am_mma(a2[x][0], brl, wr);
+      wi = am_mma(a2[x][0], bil, wi);
+      wr = am_mma(a2[x][1], brh, wr);
+      wi = am_mma(a2[x][1], bih, wi);
     }
-    const float f = (float)(p.f0 + (double)k * p.df);
+    const int kn = min(k0 + i + 1, klast);
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int hl = 0; hl < 2; ++hl) a2[x][hl] = R2[am_ridx(kn, nw, w, x, hl, lane)];
+    const float f = (float)(p.f0 + (double)(k0 + i) * p.df);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const cf Wc = {wr[c], wi[c]};
-      racc = fmaf(gr[c], Wc.x, racc);
-      racc = fmaf(-gi[c], Wc.y, racc);
+      racc = fmaf(Gi.gr[c], Wc.x, racc);
+      racc = fmaf(-Gi.gi[c], Wc.y, racc);
 #pragma unroll
       for (int j = 0; j < NSUB; ++j) {
-        const cf ew = cmul(E[c][j], Wc);
+        const cf ew = cmul(Ei[c][j], Wc);
         S0[c][j].x += ew.x;
         S0[c][j].y += ew.y;
         S1[c][j].x = fmaf(f, ew.x, S1[c][j].x);
         S1[c][j].y = fmaf(f, ew.y, S1[c][j].y);
-        E[c][j] = cmul(E[c][j], Q[c][j]);
       }
     }
+  };
+  auto advance = [&](const cf (&Ei)[4][NSUB], cf (&Eo)[4][NSUB]) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int j = 0; j < NSUB; ++j) Eo[c][j] = cmul(Ei[c][j], Q[c][j]);
+  };
+  // steady step i (2 <= i < AM_FC): Ea = E(f_{i-2}) (then E(f_{i+1})), Ec = E(f_i); Ga = G(f_{i-2}), Gc <- G(f_i)
+  auto steady = [&](int i, cf (&Ea)[4][NSUB], cf (&Ec)[4][NSUB], const AmG& Ga, AmG& Gc) {
+    stage_a(i, Ec, Gc);
+    stage_b(i - 1);
+    stage_c(i - 2, Ea, Ga);
+    advance(Ec, Ea);
+    __syncthreads();
+  };
+
+  // step 0: E(f_0) in EC, E(f_1) -> EA
+  stage_a(0, EC, GC);
+  advance(EC, EA);
+  __syncthreads();
+  // step 1: E(f_1) in EA, E(f_2) -> EB
+  stage_a(1, EA, GA);
+  stage_b(0);
+  advance(EA, EB);
+  __syncthreads();
+  // steps 2 .. AM_FC - 1, three at a time: (E(f_{i-2}), E(f_i)) = (EC, EB), (EA, EC), (EB, EA)
+  for (int i = 2; i < AM_FC; i += 3) {
+    steady(i, EC, EB, GC, GB);
+    steady(i + 1, EA, EC, GA, GC);
+    steady(i + 2, EB, EA, GB, GA);
   }
+  // drain: step AM_FC (B of the last frequency, C of the one before), step AM_FC + 1 (C of the last)
+  stage_b(AM_FC - 1);
+  stage_c(AM_FC - 2, EC, GC);
+  __syncthreads();
+  stage_c(AM_FC - 1, EA, GA);
 
   // derivative coefficients once per item; sums over the lane's terms, its 4 lane groups and the slices
   float gx = 0.f, gz = 0.f, gcs = 0.f;
@@ -645,7 +767,8 @@ __global__ __launch_bounds__(64 * AM_MAXW) void k_adjm(Problem p, int q, int nw,
   __syncthreads();
   if (threadIdx.x < 16 && valid) {
     float a[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int u = 0; u < nw; ++u)
+#pragma unroll
+    for (int u = 0; u < NW; ++u)
 #pragma unroll
       for (int i = 0; i < 4; ++i) a[i] += red[u][i][threadIdx.x];
     const float rcs = p.rc[s];
@@ -752,10 +875,10 @@ static AdjmLayout adjm_layout(const Problem& p) {
   AdjmLayout L{};
   L.nch = (p.NF + AM_FC - 1) / AM_FC;
   L.nfp = L.nch * AM_FC;
-  L.nw = (p.E + 15) / 16;
+  L.nw = p.E <= 64 ? 4 : AM_MAXW;  // element slices (waves) per workgroup: k_adjm<NSUB, 4 or 8>
   L.ngrp = (p.NT + AM_TG - 1) / AM_TG;
   const size_t tab = (size_t)p.NT * p.E * L.nfp * sizeof(cf);
-  const size_t rt = (size_t)p.NF * L.nw * 2 * 2 * 64 * sizeof(am_u32x4);
+  const size_t rt = (size_t)L.nfp * L.nw * 2 * 2 * 64 * sizeof(am_u32x4);
   const size_t part = (size_t)L.ngrp * L.nch * p.S * sizeof(float);
   size_t o = 0;
   L.oA = o;   o = al(o + tab);
@@ -791,16 +914,21 @@ static int adjoint_matrix(const Problem& p, const cf* dspect, float* d_rc, float
     hipLaunchKernelGGL(k_adjm_conj, dim3(blocks), dim3(256), 0, st, p, L.nfp, dspect, Gc);
     hipLaunchKernelGGL(k_ppc, dim3((L.nfp + 255) / 256), dim3(256), 0, st, p, L.nfp, ppc);
   }
-  const long nr = (long)p.NF * L.nw * 2 * 64;
+  const long nr = (long)L.nfp * L.nw * 2 * 64;
   const int rblocks = (int)std::min<long>((nr + 255) / 256, 65536);
   const dim3 grid((p.S + 15) / 16, L.nch);
   for (int q = 0; q < L.ngrp; ++q) {
     hipLaunchKernelGGL(k_adjm_rtab, dim3(rblocks), dim3(256), 0, st, p, L.nfp, q, L.nw, (const cf*)A,
                        (const cf*)Gc, R1, R2);
-    if (p.NS == 1)
-      hipLaunchKernelGGL(k_adjm<1>, grid, dim3(64 * L.nw), 0, st, p, q, L.nw, R1, R2, ppc, prc, px, pz, pc);
+    const dim3 block(64 * L.nw);
+    if (p.NS == 1 && L.nw == 4)
+      hipLaunchKernelGGL((k_adjm<1, 4>), grid, block, 0, st, p, q, R1, R2, ppc, prc, px, pz, pc);
+    else if (p.NS == 1)
+      hipLaunchKernelGGL((k_adjm<1, 8>), grid, block, 0, st, p, q, R1, R2, ppc, prc, px, pz, pc);
+    else if (L.nw == 4)
+      hipLaunchKernelGGL((k_adjm<2, 4>), grid, block, 0, st, p, q, R1, R2, ppc, prc, px, pz, pc);
     else
-      hipLaunchKernelGGL(k_adjm<2>, grid, dim3(64 * L.nw), 0, st, p, q, L.nw, R1, R2, ppc, prc, px, pz, pc);
+      hipLaunchKernelGGL((k_adjm<2, 8>), grid, block, 0, st, p, q, R1, R2, ppc, prc, px, pz, pc);
   }
   const int nb = L.ngrp * L.nch;
   const dim3 g1((p.S + 255) / 256);
