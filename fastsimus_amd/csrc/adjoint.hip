@@ -16,10 +16,13 @@
 // with real a_th (obliquity, directivity and 1/sqrt(r_c) terms) and complex
 // b_th (propagation terms); see coeffs_fast() below.
 //
-// k_adj_tx (lane = scatterer): P_t, B_t for one transmit, frequency blocks of
-// 32, stored to HBM; the rc gradient is reduced on the fly.
-// k_adj_pos (lane = scatterer): regenerates E_j and the derivative
+// Two implementations of the same algebra:
+// vector (any element count): k_adj_tx (lane = scatterer): P_t, B_t for one
+// transmit, frequency blocks of 32, stored to HBM; the rc gradient is reduced
+// on the fly. k_adj_pos (lane = scatterer): regenerates E_j and the derivative
 // coefficients and accumulates the x, z, c gradients against W_t.
+// matrix cores (n_elem <= 128, n_sub <= 2): k_adjm, the sums over elements as
+// split-BF16 MFMA products for 16 transmits at a time (see "matrix-core form").
 // Per-(transmit, block, scatterer) partial sums are reduced in fixed order.
 #include <algorithm>
 
