@@ -11,12 +11,12 @@ from .geometry import element_positions
 from .params import BaffleType, MediumParams, TransducerParams
 from .pfield import PfieldPlan, PfieldStrategy, pfield, pfield_compute, pfield_precompute
 from .presets import C5_2v, L11_5v, L12_3v, P4_2v
-from .simus import SimusPlan, SimusResult, SimusStrategy, simus, simus_compute, simus_precompute
+from .simus import SimusPlan, SimusResult, SimusStrategy, rf_stack, simus, simus_compute, simus_precompute
 from .tx_delay import diverging_wave, focused, plane_wave
 
 __all__ = [
     "BaffleType", "C5_2v", "L11_5v", "L12_3v", "MediumParams", "P4_2v", "PfieldPlan", "PfieldStrategy",
     "SimusPlan", "SimusResult", "SimusStrategy", "TransducerParams", "diverging_wave", "element_positions",
-    "focused", "pfield", "pfield_compute", "pfield_precompute", "plane_wave", "simus", "simus_compute",
+    "focused", "pfield", "pfield_compute", "pfield_precompute", "plane_wave", "rf_stack", "simus", "simus_compute",
     "simus_precompute",
 ]

@@ -99,4 +99,35 @@ def simus(scatterers, rc, delays, params: TransducerParams, medium: MediumParams
                          full_frequency_directivity=full_frequency_directivity, strategy=strategy)
 
 
-__all__ = ["SimusPlan", "SimusResult", "SimusStrategy", "simus", "simus_compute", "simus_precompute"]
+def rf_stack(firings, *, n_samples: int | None = None) -> torch.Tensor:
+    """RF of several firings as one ``(n_samples, n_elements, n_firings)`` array.
+
+    The layout of the reference's multi-firing harness (``tests/_picmus_phantom_shared.py:280-295``:
+    samples, channels, firings; every firing zero-padded at the end of the sample axis to the longest
+    one, and the stack padded further, never cut, to ``n_samples`` when that is larger).
+    ``firings`` is a ``(n_firings, n_samples, n_elements)`` tensor (the ``rf`` of ``simus_compute``
+    with 2-D delays, or one phantom of ``simus_batch``) or a sequence of ``(n_samples_i, n_elements)``
+    arrays. The result is float32 (or the firings' wider floating type) on the firings' device.
+    """
+    if isinstance(firings, torch.Tensor) and firings.dim() == 3:
+        items = list(firings.unbind(0))
+    else:
+        items = [f if isinstance(f, torch.Tensor) else torch.as_tensor(np.asarray(f)) for f in firings]
+    if not items:
+        raise ValueError("rf_stack needs at least one firing")
+    n_el = items[0].shape[1]
+    if any(f.dim() != 2 or f.shape[1] != n_el for f in items):
+        raise ValueError("every firing must be (n_samples, n_elements) with the same n_elements")
+    dtype = torch.promote_types(torch.float32, items[0].dtype)
+    for f in items[1:]:
+        dtype = torch.promote_types(dtype, f.dtype)
+    n = max(f.shape[0] for f in items)
+    if n_samples is not None:
+        n = max(n, int(n_samples))
+    out = torch.zeros((n, n_el, len(items)), dtype=dtype, device=items[0].device)
+    for i, f in enumerate(items):
+        out[: f.shape[0], :, i] = f.to(device=out.device, dtype=dtype)
+    return out
+
+
+__all__ = ["SimusPlan", "SimusResult", "SimusStrategy", "rf_stack", "simus", "simus_compute", "simus_precompute"]

@@ -220,15 +220,22 @@ def _adjoint_case(name):
         pos, _, _ = fs.element_positions(p.n_elements, p.pitch, p.radius)
         D = np.stack([fs.plane_wave(pos, np.deg2rad(a), speed_of_sound=1540.0) for a in np.linspace(-8, 8, 17)])
         return p, fs.MediumParams(attenuation=0.4), x, z, rc, D, 2
+    if name == "p4_phased_e64_t24":
+        p = fs.P4_2v()                                # 64 elements: four element slices per workgroup
+        x, z, rc = speckle(74, 400, xlim=(-0.04, 0.04), zlim=(0.01, 0.09))
+        pos, _, _ = fs.element_positions(p.n_elements, p.pitch, p.radius)
+        D = np.stack([fs.focused(pos, np.array([0.05 * np.sin(a), 0.05 * np.cos(a)]), speed_of_sound=1540.0)
+                      for a in np.deg2rad(np.linspace(-30, 30, 24))])
+        return p, fs.MediumParams(attenuation=0.3), x, z, rc, D, 2
     raise KeyError(name)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["l11_soft_t20", "rigid_e100_t3", "convex_impedance_t17"])
+@pytest.mark.parametrize("case", ["l11_soft_t20", "rigid_e100_t3", "convex_impedance_t17", "p4_phased_e64_t24"])
 def test_matrix_adjoint_matches_vector_adjoint(case, cuda):
     """The matrix-core adjoint against the FP32 vector adjoint on the same cotangent (more than one group
-    of 16 transmits, ragged element count, out-of-field and zero-rc scatterers, every baffle kind, a convex
-    array), and both against the forward through Euler's identity for the degree-1 homogeneous map
+    of 16 transmits, ragged element count, 64 elements, out-of-field and zero-rc scatterers, every baffle
+    kind, a convex array), and both against the forward through Euler's identity for the degree-1 homogeneous map
     rc -> X: sum_s rc_s dL/drc_s = L for L = Re sum conj(g) X."""
     from fastsimus_amd import kernels as K
 

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import numpy as np
 import pytest
+import torch
 
 import fastsimus_amd as fs
 from oracle import simus_oracle as O
@@ -93,6 +94,25 @@ def test_multi_transmit_both_methods(cuda):
     assert res.rf.shape[0] == 5 and res.spectrum.shape[0] == 5
     assert rel_l2(rf32, want) < FP32_TOL
     assert rel_l2(rfmm, want) < MFMA_TOL
+
+
+def test_rf_stack_of_shared_plan_firings(cuda):
+    """The multi-firing RF stack (samples, channels, firings) of the reference harness
+    (tests/_picmus_phantom_shared.py:319-367: one plan from the widest firing, then one simus_compute per
+    firing): simus_compute with 2-D delays and rf_stack give the same stack as per-firing calls."""
+    p = fs.L11_5v()
+    m = fs.MediumParams(attenuation=0.5)
+    x, z, rc = speckle(9, 40)
+    sc = np.stack([x, z], -1)
+    D = np.stack([fs.plane_wave(_pos(p), np.deg2rad(a), speed_of_sound=1540.0) for a in (-6, 0, 9)])
+    plan = fs.simus_precompute(sc, rc, D, p, m)
+    per = [fs.simus_compute(sc, rc, D[i], plan, p, m, strategy="hip").rf for i in range(3)]
+    stack = fs.rf_stack(fs.simus_compute(sc, rc, D, plan, p, m, strategy="hip").rf)
+    assert stack.shape == (per[0].shape[0], 128, 3) and stack.dtype == torch.float32
+    ref = fs.rf_stack(per)
+    assert torch.equal(stack, ref)
+    padded = fs.rf_stack(per, n_samples=per[0].shape[0] + 7)
+    assert padded.shape[0] == per[0].shape[0] + 7 and torch.all(padded[-7:] == 0)
 
 
 def test_phased_array_focused_beams_mfma(cuda):

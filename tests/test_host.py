@@ -252,3 +252,26 @@ def test_product_code_never_imports_the_oracle():
             else:
                 continue
             assert not any(n == "oracle" or n.startswith("oracle.") for n in names), f"{f.name} imports the oracle"
+
+
+def test_rf_stack_layout_and_padding():
+    """(samples, channels, firings), ragged firings zero-padded at the end of the sample axis, padded
+    (never cut) to n_samples, float32 or wider (tests/_picmus_phantom_shared.py:280-295)."""
+    import torch
+
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((5, 4)).astype(np.float32)
+    b = rng.standard_normal((3, 4)).astype(np.float32)
+    st = fs.rf_stack([a, b])
+    assert st.shape == (5, 4, 2) and st.dtype == torch.float32
+    assert np.array_equal(st[:, :, 0].numpy(), a)
+    assert np.array_equal(st[:3, :, 1].numpy(), b) and np.all(st[3:, :, 1].numpy() == 0)
+    assert fs.rf_stack([a, b], n_samples=8).shape == (8, 4, 2)
+    assert fs.rf_stack([a, b], n_samples=2).shape == (5, 4, 2)
+    assert fs.rf_stack([a.astype(np.float64)]).dtype == torch.float64
+    t = torch.as_tensor(rng.standard_normal((3, 6, 2)).astype(np.float32))
+    assert torch.equal(fs.rf_stack(t), t.permute(1, 2, 0))
+    with pytest.raises(ValueError):
+        fs.rf_stack([a, np.zeros((5, 3), np.float32)])
+    with pytest.raises(ValueError):
+        fs.rf_stack([])
