@@ -476,7 +476,13 @@ def main() -> int:
     plain = {"value": plain_value, "unit": UNIT, "transmits_timed": [int(i) for i in sub],
              "roofline": {"bound": "valu", "achieved": fl_plain / (kp_ms / 1e3) / 1e12, "peak": PEAK_FP32 / 1e12,
                           "unit": "TFLOP/s", "frac": fl_plain / (kp_ms / 1e3) / PEAK_FP32,
-                          "kernel": "k_tx + k_rx (FP32 VALU)", "kernel_ms_total": kp_ms, "launches": kp_n}}
+                          "kernel": "k_tx + k_rx (FP32 VALU)", "kernel_ms_total": kp_ms, "launches": kp_n,
+                          # the peak counts packed FP32 FMA, which this build does not use: the plain
+                          # kernels issue ~34.5 VALU instructions per evaluation (n_sub = 2), so their
+                          # bound is the VALU issue rate of 256 CUs x 64 lanes per cycle
+                          "valu_instructions_per_eval": 34.5 if plan.n_sub == 2 else None,
+                          "valu_issue_tlanes_per_s": (34.5 * evals_p / (kp_ms / 1e3) / 1e12
+                                                      if plan.n_sub == 2 else None)}}
     err = {"mfma_vs_fp32_full": rel_l2(rf_m[sub].cpu().numpy(), rf_p.cpu().numpy())}
 
     # FP64 oracle on a scatterer subsample: CPU baseline and absolute error of both GPU paths
