@@ -293,6 +293,120 @@ def bench_adjoint(args) -> int:
     return 0
 
 
+def single_workloads(seed: int = 2025):
+    """The two single-transmit workloads: 2 000 speckle scatterers, zero delays (the CPU-runnable case), and
+    50 000 speckle scatterers plus 10 bright points (x = 0, z = 5 .. 50 mm, rc = 10) under a 0 degree plane
+    wave; 128-element 7.6 MHz linear array, 0.5 dB/cm/MHz, inputs drawn in float64 and rounded once to float32."""
+    import fastsimus_amd as fs
+
+    params = fs.TransducerParams(freq_center=7.6e6, pitch=3.0e-4, width=2.7e-4, n_elements=128, bandwidth=0.77)
+    medium = fs.MediumParams(speed_of_sound=1540.0, attenuation=0.5)
+    pos, _, _ = fs.element_positions(params.n_elements, params.pitch, params.radius)
+    rng = np.random.default_rng(seed)
+    x0 = rng.uniform(-0.02, 0.02, 2000).astype(np.float32)
+    z0 = rng.uniform(0.005, 0.045, 2000).astype(np.float32)
+    r0 = rng.standard_normal(2000).astype(np.float32)
+    x1 = np.concatenate([rng.uniform(-0.02, 0.02, 50_000), np.zeros(10)]).astype(np.float32)
+    z1 = np.concatenate([rng.uniform(0.005, 0.05, 50_000), np.linspace(0.005, 0.05, 10)]).astype(np.float32)
+    r1 = np.concatenate([rng.standard_normal(50_000), np.full(10, 10.0)]).astype(np.float32)
+    d0 = np.zeros(params.n_elements)
+    d1 = fs.plane_wave(pos, 0.0, speed_of_sound=medium.speed_of_sound)
+    return params, medium, (x0, z0, r0, d0), (x1, z1, r1, d1)
+
+
+def bench_single(args) -> int:
+    """Single-transmit workloads (one step = spectrum + RF of one simus_compute, inputs in HBM): the
+    2 000-scatterer case against the FP64 NumPy oracle in full, and the 50 010-scatterer case with the
+    plain FP32 kernels (FP64 error on a 1 010-scatterer sample: 1 000 speckle + the 10 points)."""
+    import torch
+    from threadpoolctl import threadpool_limits
+
+    import fastsimus_amd as fs
+    from fastsimus_amd import kernels as K
+    from oracle import simus_oracle as O
+
+    dev = torch.device("cuda", 0)
+    params, medium, c0, c1 = single_workloads()
+
+    def problem(x, z, rc, d, plan):
+        sc = np.stack([x, z], -1)
+        return K.Problem(sc, rc, d, None, params, medium, n_sub=plan.n_sub, seg_length=plan.seg_length,
+                         freqs=plan.selected_freqs, pulse=plan.pulse_spectrum, probe=plan.probe_spectrum,
+                         correction=plan.correction_factor, full_frequency_directivity=False, device=dev)
+
+    def timed(prob, plan, method):
+        def step():
+            sp = K.run_spectrum(prob, method)
+            return K.run_rf(sp, plan.n_freq_full, plan.freq_idx_start, plan.n_fft, want_full=False)[0]
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            rf = step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps, rf[0].cpu().numpy()
+
+    out = {}
+    # 2 000 scatterers: both device methods, FP64 oracle in full on one core
+    x, z, rc, d = c0
+    plan = fs.simus_precompute(np.stack([x, z], -1), rc, d, params, medium)
+    plan_o = O.simus_plan(x.astype(np.float64), z.astype(np.float64), d, params, medium)
+    prob = problem(x, z, rc, d, plan)
+    evals = x.size * params.n_elements * plan.selected_freqs.shape[0]
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        want, _, _ = O.simus_oracle(x.astype(np.float64), z.astype(np.float64), rc.astype(np.float64), d, params,
+                                    medium, plan_o)
+        t_cpu = time.perf_counter() - t0
+    r = {"n_scat": int(x.size), "n_freq": int(plan.selected_freqs.shape[0]),
+         "cpu_baseline": {"value": evals / t_cpu / 1e9, "unit": UNIT, "cores": 1, "kind": "port",
+                          "sample": f"all {x.size} scatterers, FP64 NumPy, {t_cpu:.2f} s"}}
+    for method in ("fp32", "mfma"):
+        t, rf = timed(prob, plan, method)
+        r[method] = {"value": evals / t / 1e9, "ms_per_step": t * 1e3, "rf_rel_l2_vs_fp64": rel_l2(rf, want)}
+    out["2000_scatterers_zero_delays"] = r
+    # 50 010 scatterers: plain FP32 kernels; FP64 error on a sample of the same plan
+    x, z, rc, d = c1
+    plan = fs.simus_precompute(np.stack([x, z], -1), rc, d, params, medium)
+    plan_o = O.simus_plan(x.astype(np.float64), z.astype(np.float64), d, params, medium)
+    evals = x.size * params.n_elements * plan.selected_freqs.shape[0]
+    r = {"n_scat": int(x.size), "n_freq": int(plan.selected_freqs.shape[0])}
+    for method in ("fp32", "mfma"):
+        t, _ = timed(problem(x, z, rc, d, plan), plan, method)
+        r[method] = {"value": evals / t / 1e9, "ms_per_step": t * 1e3}
+    idx = np.concatenate([np.sort(np.random.default_rng(3).choice(50_000, 1000, replace=False)),
+                          np.arange(50_000, 50_010)])
+    xs, zs, rs = x[idx], z[idx], rc[idx]
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        want, _, _ = O.simus_oracle(xs.astype(np.float64), zs.astype(np.float64), rs.astype(np.float64), d, params,
+                                    medium, plan_o)
+        t_cpu = time.perf_counter() - t0
+    for method in ("fp32", "mfma"):
+        _, rf = timed(problem(xs, zs, rs, d, plan), plan, method)
+        r[method]["rf_rel_l2_vs_fp64"] = rel_l2(rf, want)
+    ev_s = idx.size * params.n_elements * plan_o["freqs"].size
+    r["fp64_sample"] = "1 010 scatterers (1 000 speckle + the 10 points), same plan"
+    r["cpu_baseline"] = {"value": ev_s / t_cpu / 1e9, "unit": UNIT, "cores": 1, "kind": "port",
+                         "sample": f"the 1 010-scatterer sample, FP64 NumPy, {t_cpu:.2f} s"}
+    out["50010_scatterers_plane_wave"] = r
+    main_r = out["50010_scatterers_plane_wave"]["fp32"]
+    res = {
+        "metric": "G(scatterer·element·freq)/s, single transmit (plain FP32 and split-BF16 MFMA)",
+        "value": main_r["value"], "unit": UNIT, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": main_r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "single transmit, 128-element 7.6 MHz linear array, 0.5 dB/cm/MHz: 2 000 speckle "
+                               "scatterers with zero delays, and 50 000 speckle + 10 bright points under a 0 degree "
+                               "plane wave (value: the latter, plain FP32)", "n_elem": params.n_elements},
+        "configs": out, "roofline": None,
+        "cpu_baseline": out["50010_scatterers_plane_wave"]["cpu_baseline"],
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def bench_pfield(args) -> int:
     """RMS pressure field of one focused transmit of the 128-element linear array on a 512 x 512 grid
     (x in [-2, 2] cm, z in [0.2, 5] cm); one step = fs_pfield over the grid, inputs in HBM."""
@@ -373,10 +487,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint", choices=("auto", "matrix", "vector"), default="auto",
                     help="adjoint15: backward kernels (matrix cores where they apply, or FP32 vector kernels)")
-    ap.add_argument("--workload", choices=("planewave75", "phantoms64", "adjoint15", "pfield"), default="planewave75",
+    ap.add_argument("--workload", choices=("planewave75", "phantoms64", "adjoint15", "pfield", "single"),
+                    default="planewave75",
                     help="planewave75 (default, the headline metric), phantoms64 (uneven phantom batch, phased "
                          "array, focused beams), adjoint15 (forward + gradient of the differentiable form) or "
-                         "pfield (RMS pressure field on a grid)")
+                         "pfield (RMS pressure field on a grid) or single (the single-transmit 2 000- and "
+                         "50 010-scatterer cases)")
     ap.add_argument("--n-phantoms", type=int, default=64)
     ap.add_argument("--tune", action="store_true",
                     help="run the tile-shape autotuner before timing (default: library defaults, so that a kernel "
@@ -410,6 +526,8 @@ def main() -> int:
         return bench_phantoms(args)
     if args.workload == "adjoint15":
         return bench_adjoint(args)
+    if args.workload == "single":
+        return bench_single(args)
     if args.workload == "pfield":
         return bench_pfield(args)
     params, medium, x, z, rc, delays = workload(n_scat=args.n_scat, n_tx=args.n_tx)
