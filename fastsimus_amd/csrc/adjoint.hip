@@ -366,6 +366,9 @@ constexpr int AM_TG = 16;    // transmits per group
 constexpr int AM_MAXW = 8;   // element slices per workgroup (E <= 128)
 constexpr int AM_NSUB_MAX = 2;
 constexpr int AM_YROW = 16;  // LDS row of Y: 16 scatterers
+// Y rows are stored with their four 4-float groups XOR-swizzled by (row % 16) / 4, so that the
+// 16-byte stores of 16 lanes (16 consecutive rows) and the 4-byte loads of the reduction (64
+// consecutive floats of 4 rows) each fall on 64 distinct banks.
 
 typedef __bf16 am_bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 am_bf16x2 __attribute__((ext_vector_type(2)));
@@ -529,8 +532,8 @@ __device__ __forceinline__ void am_reduce(const float* ypr, const float* ypi, un
     float tr[NW], ti[NW];
 #pragma unroll
     for (int v = 0; v < NW; ++v) {
-      tr[v] = ypr[(v * 32 + m) * AM_YROW + sl];
-      ti[v] = ypi[(v * 32 + m) * AM_YROW + sl];
+      tr[v] = ypr[(v * 32 + m) * AM_YROW + (sl ^ (4 * ((m & 15) >> 2)))];
+      ti[v] = ypi[(v * 32 + m) * AM_YROW + (sl ^ (4 * ((m & 15) >> 2)))];
     }
     float yx = 0.f, yy = 0.f;
 #pragma unroll
@@ -653,8 +656,8 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, const am_u32
     float* ib = ypi + (i & 1) * YB;
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
-      *(am_floatx4*)(yb + (w * 32 + 16 * x + col) * AM_YROW + 4 * g) = yr[x];
-      *(am_floatx4*)(ib + (w * 32 + 16 * x + col) * AM_YROW + 4 * g) = yi[x];
+      *(am_floatx4*)(yb + (w * 32 + 16 * x + col) * AM_YROW + 4 * (g ^ (col >> 2))) = yr[x];
+      *(am_floatx4*)(ib + (w * 32 + 16 * x + col) * AM_YROW + 4 * (g ^ (col >> 2))) = yi[x];
     }
   };
   // stage B(i): L of f_i from Y buffer i % 2 into L buffer i % 2
