@@ -758,10 +758,13 @@ __global__ void k_set_int(int* __restrict__ ptr, int value) {
 constexpr int AP_CH = 32;  // e' per LDS chunk
 
 // Grid (frequency, phantom); phantom ph sums its K slots and writes out[ph][t][k][e].
+// MAXT transmits per thread and pass (2 MAXT per pass over the two thread parities): 40, or fewer
+// when the problem has fewer transmits, so that the per-thread accumulation does no padded work.
+template <int MAXT>
 __global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, const MsmPhantoms phantoms, int nfpad,
                                                    const cf* __restrict__ kpart, cf* __restrict__ out_all) {
   __shared__ cf Ks[128][AP_CH + 1];
-  __shared__ cf As[AP_CH][2 * 40 + 1];
+  __shared__ cf As[AP_CH][2 * MAXT + 1];
   const int k = blockIdx.x;
   const int slot0 = phantoms.ph[blockIdx.y].x, nchunks = phantoms.ph[blockIdx.y].y;
   cf* __restrict__ out = out_all + (size_t)blockIdx.y * p.NT * p.NF * p.E;
@@ -769,7 +772,6 @@ __global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, const M
   const int e = tid & 127;
   const int tg = tid >> 7;  // transmit parity
   const double f = p.f0 + (double)k * p.df;
-  constexpr int MAXT = 40;  // transmits per pass per thread parity (80 per pass)
   for (int t0 = 0; t0 < p.NT; t0 += 2 * MAXT) {
     cf acc[MAXT];
 #pragma unroll
@@ -1097,7 +1099,13 @@ int mfma_spectrum_batch(const Problem& p, int P, const int64_t* off, cf* spect, 
     default: return FS_ERR_UNSUPPORTED;
   }
   if (rc != FS_OK) return rc;
-  hipLaunchKernelGGL(k_msm_apply, dim3(p.NF, P), dim3(256), 0, st, p, m.ET * 16, m.phantoms, m.nfpad, kpart, spect);
+  const dim3 ag(p.NF, P);
+  if (p.NT <= 2)
+    hipLaunchKernelGGL(k_msm_apply<1>, ag, dim3(256), 0, st, p, m.ET * 16, m.phantoms, m.nfpad, kpart, spect);
+  else if (p.NT <= 16)
+    hipLaunchKernelGGL(k_msm_apply<8>, ag, dim3(256), 0, st, p, m.ET * 16, m.phantoms, m.nfpad, kpart, spect);
+  else
+    hipLaunchKernelGGL(k_msm_apply<40>, ag, dim3(256), 0, st, p, m.ET * 16, m.phantoms, m.nfpad, kpart, spect);
   return hipGetLastError() == hipSuccess ? FS_OK : FS_ERR_LAUNCH;
 }
 
