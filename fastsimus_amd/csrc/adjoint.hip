@@ -351,10 +351,11 @@ __global__ void k_ppc(Problem p, int nfp, cf* __restrict__ ppc) {
 //   S0_j = sum_f E_j W, S1_j = sum_f f E_j W      (as in k_adj_pos)
 // Y and W^T are split-BF16 MFMA products (hi*hi + hi*lo + lo*hi, FP32
 // accumulation) with the complex parts in the 32-deep contraction
-// ([Re | Im] of 16 complex indices; the A operand negated in its upper half
-// for the real part, halves swapped for the imaginary part). G is generated
-// once per (scatterer, element, frequency) for all transmits of a group
-// instead of once per transmit.
+// ([Re | Im] of 16 complex indices). For the real part one operand is negated
+// in its upper half, for the imaginary part its halves are swapped: G (the A
+// operand) in Y, L^T (the B operand, written in both forms by the reduction)
+// in W^T. G is generated once per (scatterer, element, frequency) for all
+// transmits of a group instead of once per transmit.
 //
 // Workgroup: 16 scatterers x nw element slices (one wave per 16 elements),
 // AM_FC frequencies. Lane l of wave w holds scatterer s0 + l % 16 and elements
