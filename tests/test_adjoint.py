@@ -227,11 +227,18 @@ def _adjoint_case(name):
         D = np.stack([fs.focused(pos, np.array([0.05 * np.sin(a), 0.05 * np.cos(a)]), speed_of_sound=1540.0)
                       for a in np.deg2rad(np.linspace(-30, 30, 24))])
         return p, fs.MediumParams(attenuation=0.3), x, z, rc, D, 2
+    if name == "tiny_e7_t1":
+        p = fs.TransducerParams(freq_center=5e6, pitch=3.0e-4, width=2.6e-4, n_elements=7, bandwidth=0.6)
+        x, z, rc = speckle(75, 37, xlim=(-0.004, 0.004), zlim=(0.002, 0.02))
+        pos, _, _ = fs.element_positions(p.n_elements, p.pitch, p.radius)
+        D = fs.plane_wave(pos, np.deg2rad(3.0), speed_of_sound=1540.0)[None, :]
+        return p, fs.MediumParams(attenuation=0.2), x, z, rc, D, 2
     raise KeyError(name)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["l11_soft_t20", "rigid_e100_t3", "convex_impedance_t17", "p4_phased_e64_t24"])
+@pytest.mark.parametrize("case", ["l11_soft_t20", "rigid_e100_t3", "convex_impedance_t17", "p4_phased_e64_t24",
+                                  "tiny_e7_t1"])
 def test_matrix_adjoint_matches_vector_adjoint(case, cuda):
     """The matrix-core adjoint against the FP32 vector adjoint on the same cotangent (more than one group
     of 16 transmits, ragged element count, 64 elements, out-of-field and zero-rc scatterers, every baffle
