@@ -275,3 +275,13 @@ def test_rf_stack_layout_and_padding():
         fs.rf_stack([a, np.zeros((5, 3), np.float32)])
     with pytest.raises(ValueError):
         fs.rf_stack([])
+
+
+def test_adjoint_method_switch_validates():
+    """fs_set_adjoint_method takes 0 (auto), 1 (matrix cores), 2 (vector kernels); anything else is
+    FS_ERR_INVALID and leaves the choice unchanged. No GPU call is made."""
+    lib = _lib.load()
+    assert lib.fs_set_adjoint_method(3) == _lib.FS_ERR_INVALID
+    assert lib.fs_set_adjoint_method(-1) == _lib.FS_ERR_INVALID
+    for m in (1, 2, 0):
+        assert lib.fs_set_adjoint_method(m) == 0
