@@ -29,7 +29,11 @@ Pinning (what checks each function, see ``tests/test_oracle.py``):
   and against the single-element single-scatterer closed form.
 * ``rf_from_spectrum``: the inverse-DFT definition, the zero-input case and
   the echo-arrival-time invariant.
-* ``pfield_oracle``: brute force of the same sum.
+* ``pfield_oracle`` / ``pfield_plan``: two special cases that reduce to closed
+  forms with the geometry computed by hand in the test (one element, rigid
+  baffle, no attenuation: D / sqrt(r) * sqrt(corr sum |PP|^2); two elements and
+  a delay dt on the axis: the 2 + 2 cos(2 pi f dt) interference), and the
+  on-axis focal peak.
 * the adjoint oracle (``simus_loss_grad_fp64``): central finite differences
   of the oracle loss itself.
 """

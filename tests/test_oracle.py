@@ -267,3 +267,47 @@ def test_pfield_on_axis_focus_peak():
     pr = O.pfield_oracle(x, z, d, p, fs.MediumParams(), plan)
     assert 0.012 < z[np.argmax(pr)] < 0.028
     assert np.all(pr > 0)
+
+
+def _pp_power(plan):
+    return float(np.sum(np.abs(plan["pulse"] * plan["probe"]) ** 2))
+
+
+def test_pfield_single_element_closed_form():
+    """One element, one sub-element, rigid baffle, no attenuation, zero delay: |G A| = D / sqrt(r) at every
+    frequency (D the centre-frequency sinc directivity), so the RMS field reduces to
+    D / sqrt(r) * sqrt(corr * sum_f |pulse_f probe_f|^2). Geometry computed here by hand."""
+    p = fs.TransducerParams(freq_center=5e6, pitch=3.0e-4, width=2.7e-4, n_elements=1, bandwidth=0.6,
+                            baffle="rigid")
+    m = fs.MediumParams(speed_of_sound=1540.0, attenuation=0.0)
+    x, z = np.array([0.002, -0.001]), np.array([0.015, 0.004])
+    d = np.zeros(1)
+    plan = O.pfield_plan(x, z, d, p, m, element_splitting=1)
+    got = O.pfield_oracle(x, z, d, p, m, plan)
+    r = np.hypot(x, z)
+    kc = 2 * np.pi * p.freq_center / m.speed_of_sound
+    D = np.sinc(kc * p.element_width / 2 * (x / r) / np.pi)
+    want = np.abs(D) / np.sqrt(r) * np.sqrt(plan["corr"] * _pp_power(plan))
+    assert np.allclose(got, want, rtol=1e-12, atol=0)
+
+
+def test_pfield_two_element_interference_closed_form():
+    """Two elements at +-pitch/2 (float32 positions, as the reference lays them out), a point on the axis and
+    delays (0, dt): both terms have the same amplitude a, so |G1 A1 + G2 A2|^2 = a^2 (2 + 2 cos(2 pi f dt)) and
+    the RMS field is a * sqrt(corr * sum_f |PP_f|^2 (2 + 2 cos(2 pi f dt)))."""
+    p = fs.TransducerParams(freq_center=5e6, pitch=3.0e-4, width=2.7e-4, n_elements=2, bandwidth=0.6,
+                            baffle="rigid")
+    m = fs.MediumParams(speed_of_sound=1540.0, attenuation=0.0)
+    x, z = np.array([0.0]), np.array([0.012])
+    dt = 50e-9
+    d = np.array([0.0, dt])
+    plan = O.pfield_plan(x, z, d, p, m, element_splitting=1)
+    got = O.pfield_oracle(x, z, d, p, m, plan)
+    xe = float(np.float32(0.5) * np.float32(p.pitch))
+    r = np.hypot(xe, z[0])
+    kc = 2 * np.pi * p.freq_center / m.speed_of_sound
+    a = abs(np.sinc(kc * p.element_width / 2 * (xe / r) / np.pi)) / np.sqrt(r)
+    f = plan["freqs"]
+    pw = np.sum(np.abs(plan["pulse"] * plan["probe"]) ** 2 * (2 + 2 * np.cos(2 * np.pi * f * dt)))
+    want = a * np.sqrt(plan["corr"] * pw)
+    assert got[0] == pytest.approx(want, rel=1e-12)
