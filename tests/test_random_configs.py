@@ -1,0 +1,95 @@
+"""Seeded random configurations through both device methods against the FP64 oracle.
+
+Each case draws an array (1 to 160 elements, linear or convex, any baffle kind), a medium, a small
+scatterer set (some out of the field), one to three transmits (plane waves or focused beams, some
+elements with NaN delays), the sub-element count and the directivity option. The plain FP32 method
+must match the oracle's RF to 1e-4 relative L2 and the split-BF16 method (n_elem <= 128) to 1e-3,
+the same bars as the fixed cases.
+"""
+
+from __future__ import annotations
+
+from math import inf
+
+import numpy as np
+import pytest
+
+import fastsimus_amd as fs
+from oracle import simus_oracle as O
+from tests.conftest import f32, rel_l2
+
+SEEDS = list(range(48))
+
+
+def random_case(seed: int):
+    rng = np.random.default_rng(1000 + seed)
+    convex = rng.random() < 0.3
+    E = int(rng.choice([1, 5, 16, 33, 64, 100, 128, 160]))
+    fc = float(rng.uniform(2e6, 9e6))
+    c = float(rng.uniform(1450.0, 1600.0))
+    pitch = float(rng.uniform(0.8, 1.6) * c / fc / 2)
+    width = pitch * float(rng.uniform(0.6, 0.95))
+    kind = str(rng.choice(["soft", "rigid", "impedance"]))
+    baffle = float(rng.uniform(0.5, 3.0)) if kind == "impedance" else kind
+    radius = float(rng.uniform(0.02, 0.08)) if convex else inf
+    params = fs.TransducerParams(freq_center=fc, pitch=pitch, width=width, n_elements=E,
+                                 bandwidth=float(rng.uniform(0.5, 0.9)), radius=radius, baffle=baffle)
+    medium = fs.MediumParams(speed_of_sound=c, attenuation=float(rng.uniform(0.0, 1.0)))
+    S = int(rng.integers(1, 25))
+    half = max(pitch * E, 0.004)
+    x = f32(rng.uniform(-half, half, S))
+    z = f32(rng.uniform(-0.002, 0.03, S))          # z < 0: out of the field
+    rc = f32(rng.standard_normal(S))
+    pos = fs.element_positions(E, pitch, radius)[0]
+    T = int(rng.integers(1, 4))
+    rows = []
+    for _ in range(T):
+        if rng.random() < 0.5 and not convex:
+            rows.append(fs.plane_wave(pos, float(np.deg2rad(rng.uniform(-15, 15))), speed_of_sound=c))
+        else:
+            foc = np.array([rng.uniform(-half, half), rng.uniform(0.005, 0.03)])
+            rows.append(fs.focused(pos, foc, speed_of_sound=c))
+    D = np.stack(rows)
+    if rng.random() < 0.25 and E > 2:
+        D[:, rng.choice(E, max(1, E // 10), replace=False)] = np.nan
+    kw = {}
+    nsub = rng.choice([0, 1, 2, 3])
+    if nsub:
+        kw["element_splitting"] = int(nsub)
+    if rng.random() < 0.2:
+        kw["full_frequency_directivity"] = True
+    return params, medium, x, z, rc, D, kw
+
+
+def _oracle_rf(x, z, rc, D, params, medium, kw):
+    plan = O.simus_plan(x, z, D, params, medium, element_splitting=kw.get("element_splitting"))
+    fd = kw.get("full_frequency_directivity", False)
+    return np.stack([O.simus_oracle(x, z, rc, d, params, medium, plan, full_frequency_directivity=fd)[0] for d in D])
+
+
+def test_random_cases_are_valid_on_the_host():
+    """The generator only draws inputs the public API accepts (no GPU needed)."""
+    for seed in SEEDS:
+        params, medium, x, z, rc, D, kw = random_case(seed)
+        plan = fs.simus_precompute(np.stack([x, z], -1), rc, D, params, medium,
+                                   element_splitting=kw.get("element_splitting"))
+        assert plan.selected_freqs.size > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", SEEDS)
+def test_random_configuration_matches_oracle(seed, cuda):
+    params, medium, x, z, rc, D, kw = random_case(seed)
+    sc = np.stack([x, z], -1)
+    want = _oracle_rf(x, z, rc, D, params, medium, kw)
+    print(f"seed {seed}: E={params.n_elements} T={D.shape[0]} S={x.size} convex={params.radius != inf} "
+          f"baffle={params.baffle} {kw}", flush=True)
+    rf = fs.simus(sc, rc, D, params, medium, strategy="hip", **kw).rf.cpu().numpy().reshape(want.shape)
+    e32 = rel_l2(rf, want)
+    print(f"  plain FP32 rel L2 {e32:.2e}", flush=True)
+    assert e32 < 1e-4
+    if params.n_elements <= 128:
+        rf = fs.simus(sc, rc, D, params, medium, strategy="hip_mfma", **kw).rf.cpu().numpy().reshape(want.shape)
+        em = rel_l2(rf, want)
+        print(f"  split-BF16 rel L2 {em:.2e}", flush=True)
+        assert em < 1e-3
