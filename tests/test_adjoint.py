@@ -16,7 +16,7 @@ import torch
 import fastsimus_amd as fs
 from oracle import simus_oracle as O
 from oracle.simus_grad import simus_rf_torch
-from tests.conftest import rel_l2, speckle
+from tests.conftest import f32, rel_l2, speckle
 
 PARAMS = fs.TransducerParams(freq_center=7.6e6, pitch=3.0e-4, width=2.7e-4, n_elements=128, bandwidth=0.77)
 ATT = 0.5
@@ -269,3 +269,69 @@ def test_matrix_adjoint_matches_vector_adjoint(case, cuda):
     assert np.all(mat[1][rc == 0] == 0) and np.all(mat[2][rc == 0] == 0)
     for d_rc in (mat[0], vec[0]):
         assert float((rc * d_rc).sum()) == pytest.approx(L, rel=1e-3)
+
+
+def _random_adjoint_case(seed):
+    """A random configuration the matrix-core adjoint covers (n_elem <= 128, n_sub <= 2), 1-20 transmits."""
+    from math import inf
+
+    rng = np.random.default_rng(5000 + seed)
+    convex = rng.random() < 0.3
+    E = int(rng.choice([1, 5, 16, 33, 64, 100, 128]))
+    fc = float(rng.uniform(2e6, 9e6))
+    c = float(rng.uniform(1450.0, 1600.0))
+    pitch = float(rng.uniform(0.8, 1.6) * c / fc / 2)
+    kind = str(rng.choice(["soft", "rigid", "impedance"]))
+    p = fs.TransducerParams(freq_center=fc, pitch=pitch, width=pitch * float(rng.uniform(0.6, 0.95)), n_elements=E,
+                            bandwidth=float(rng.uniform(0.5, 0.9)),
+                            radius=float(rng.uniform(0.02, 0.08)) if convex else inf,
+                            baffle=float(rng.uniform(0.5, 3.0)) if kind == "impedance" else kind)
+    m = fs.MediumParams(speed_of_sound=c, attenuation=float(rng.uniform(0.0, 1.0)))
+    S = int(rng.integers(1, 60))
+    half = max(pitch * E, 0.004)
+    x = f32(rng.uniform(-half, half, S))
+    z = f32(rng.uniform(-0.002, 0.03, S))
+    rc = f32(rng.standard_normal(S))
+    rc[rng.random(S) < 0.1] = 0.0
+    pos = fs.element_positions(E, pitch, p.radius)[0]
+    T = int(rng.integers(1, 21))
+    D = np.stack([fs.focused(pos, np.array([rng.uniform(-half, half), rng.uniform(0.005, 0.03)]), speed_of_sound=c)
+                  for _ in range(T)])
+    if rng.random() < 0.25 and E > 2:
+        D[:, rng.choice(E, max(1, E // 10), replace=False)] = np.nan
+    return p, m, x, z, rc, D, int(rng.choice([1, 2]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", list(range(24)))
+def test_random_adjoint_matrix_matches_vector(seed, cuda):
+    """Seeded random configurations: the matrix-core and the vector adjoint give the same gradients, and
+    both satisfy sum_s rc_s dL/drc_s = L for a linear functional of the spectrum."""
+    from fastsimus_amd import kernels as K
+
+    p, m, x, z, rc, D, nsub = _random_adjoint_case(seed)
+    print(f"seed {seed}: E={p.n_elements} T={D.shape[0]} S={x.size} n_sub={nsub} baffle={p.baffle} "
+          f"convex={p.radius != float('inf')}", flush=True)
+    sc = np.stack([x, z], -1)
+    plan = fs.simus_precompute(sc, rc, D, p, m, element_splitting=nsub)
+    prob = K.Problem(sc, rc, D, None, p, m, n_sub=plan.n_sub, seg_length=plan.seg_length, freqs=plan.selected_freqs,
+                     pulse=plan.pulse_spectrum, probe=plan.probe_spectrum, correction=plan.correction_factor,
+                     full_frequency_directivity=False, device=cuda)
+    X = torch.view_as_real(K.run_spectrum(prob, "fp32")).double().cpu().numpy()
+    g = np.random.default_rng(seed).standard_normal(X.shape).astype(np.float32)
+    L = float((g.astype(np.float64) * X).sum())
+    gt = torch.as_tensor(g, device=cuda)
+    mat = _spectral_adjoint(prob, gt, "matrix")
+    vec = _spectral_adjoint(prob, gt, "vector")
+    for name, a, b in zip(("rc", "x", "z"), mat[:3], vec[:3]):
+        if np.linalg.norm(b) > 0:
+            err = rel_l2(a, b)
+            print(f"  {name} matrix vs vector rel L2 {err:.2e}", flush=True)
+            assert err < 1e-3, name
+        else:
+            assert np.linalg.norm(a) == 0
+    if vec[3] != 0:
+        assert mat[3] == pytest.approx(vec[3], rel=1e-3)
+    if abs(L) > 0:
+        for d_rc in (mat[0], vec[0]):
+            assert float((rc * d_rc).sum()) == pytest.approx(L, rel=1e-3, abs=1e-6 * float(np.abs(X).sum()))
