@@ -103,6 +103,33 @@ def test_batch_equals_single_phantom_calls(cuda):
 
 
 @pytest.mark.gpu
+def test_batch_above_one_launch_is_split(cuda):
+    """More phantoms than one launch holds (192): the batch is split over launches and every phantom,
+    including the ones of the second launch and empty ones, equals its own call under the shared plan."""
+    p = fs.P4_2v()
+    m = fs.MediumParams(attenuation=0.3)
+    rng = np.random.default_rng(77)
+    sizes = rng.integers(0, 24, 200)
+    sizes[[0, 150, 199]] = 0
+    ph = []
+    for n in sizes:
+        sc = np.stack([rng.uniform(-0.02, 0.02, n), rng.uniform(0.01, 0.06, n)], -1).astype(np.float32)
+        ph.append((sc, rng.standard_normal(n).astype(np.float32)))
+    pos = fs.element_positions(p.n_elements, p.pitch, p.radius)[0]
+    D = fs.focused(pos, np.array([0.0, 0.04]), speed_of_sound=1540.0)[None, :]
+    plan = B.simus_batch_precompute(ph, D, p, m)
+    rf = B.simus_batch_compute(ph, D, plan, p, m, strategy="hip_mfma").rf.cpu().numpy()
+    assert rf.shape[0] == 200
+    for q in (0, 1, 150, 191, 192, 193, 198, 199):
+        sc, rc = ph[q]
+        if sc.shape[0] == 0:
+            assert not np.any(rf[q])
+            continue
+        one = fs.simus_compute(sc, rc, D, plan, p, m, strategy="hip_mfma").rf.cpu().numpy()
+        assert rel_l2(rf[q], one) < 1e-5 or not np.any(one), q
+
+
+@pytest.mark.gpu
 def test_phased_array_batch_focused_beams(cuda):
     """64-element phased array (one sub-element), focused beams over a sector, uneven phantoms."""
     p = fs.P4_2v()
