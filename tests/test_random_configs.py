@@ -93,3 +93,22 @@ def test_random_configuration_matches_oracle(seed, cuda):
         em = rel_l2(rf, want)
         print(f"  split-BF16 rel L2 {em:.2e}", flush=True)
         assert em < 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", list(range(24)))
+def test_random_pfield_matches_oracle(seed, cuda):
+    """pfield on random arrays, media and grids (points behind the array included) against the FP64 oracle."""
+    params, medium, _, _, _, D, kw = random_case(seed)
+    rng = np.random.default_rng(2000 + seed)
+    half = max(params.pitch * params.n_elements, 0.004)
+    g = f32(np.stack([rng.uniform(-half, half, 60), rng.uniform(-0.002, 0.03, 60)], -1))
+    d = D[0]
+    print(f"seed {seed}: E={params.n_elements} convex={params.radius != inf} baffle={params.baffle} {kw}", flush=True)
+    pf = fs.pfield(g, d, params, medium, **kw).cpu().numpy()
+    pl = O.pfield_plan(g[:, 0], g[:, 1], d, params, medium, element_splitting=kw.get("element_splitting"))
+    want = O.pfield_oracle(g[:, 0], g[:, 1], d, params, medium, pl,
+                           full_frequency_directivity=kw.get("full_frequency_directivity", False))
+    e = rel_l2(pf, want)
+    print(f"  pfield rel L2 {e:.2e}", flush=True)
+    assert e < 1e-4
