@@ -112,3 +112,29 @@ def test_random_pfield_matches_oracle(seed, cuda):
     e = rel_l2(pf, want)
     print(f"  pfield rel L2 {e:.2e}", flush=True)
     assert e < 1e-4
+
+
+# the first 8 draws from seed 100 on whose array the split-BF16 method takes (at most 128 elements)
+MEDIUM_SEEDS = [s for s in range(100, 200) if random_case(s)[0].n_elements <= 128][:8]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", MEDIUM_SEEDS)
+def test_random_medium_size_methods_agree(seed, cuda):
+    """Thousands of scatterers (several work units and chunks of the persistent launch) on random arrays
+    with up to 128 elements: the split-BF16 method against the plain FP32 one (the FP64 oracle is too slow
+    here; both are pinned against it on the small cases)."""
+    params, medium, _, _, _, D, kw = random_case(seed)
+    rng = np.random.default_rng(3000 + seed)
+    S = int(rng.integers(2000, 20000))
+    half = max(params.pitch * params.n_elements, 0.004)
+    x = f32(rng.uniform(-half, half, S))
+    z = f32(rng.uniform(-0.002, 0.03, S))
+    rc = f32(rng.standard_normal(S))
+    sc = np.stack([x, z], -1)
+    print(f"seed {seed}: E={params.n_elements} T={D.shape[0]} S={S} {kw}", flush=True)
+    a = fs.simus(sc, rc, D, params, medium, strategy="hip", **kw).rf.cpu().numpy()
+    b = fs.simus(sc, rc, D, params, medium, strategy="hip_mfma", **kw).rf.cpu().numpy()
+    e = rel_l2(b, a)
+    print(f"  split-BF16 vs plain FP32 rel L2 {e:.2e}", flush=True)
+    assert e < 1e-4
