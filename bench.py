@@ -503,7 +503,7 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         return 2
 
     import torch
@@ -595,7 +595,7 @@ def main() -> int:
              "roofline": {"bound": "valu", "achieved": fl_plain / (kp_ms / 1e3) / 1e12, "peak": PEAK_FP32 / 1e12,
                           "unit": "TFLOP/s", "frac": fl_plain / (kp_ms / 1e3) / PEAK_FP32,
                           "kernel": "k_tx + k_rx (FP32 VALU)", "kernel_ms_total": kp_ms, "launches": kp_n,
-                          # the peak counts packed FP32 FMA, which this build does not use: the plain
+                          # the peak counts packed FP32 FMA, which the kernels do not use: the plain
                           # kernels issue ~34.5 VALU instructions per evaluation (n_sub = 2), so their
                           # bound is the VALU issue rate of 256 CUs x 64 lanes per cycle
                           "valu_instructions_per_eval": 34.5 if plan.n_sub == 2 else None,

@@ -63,7 +63,7 @@ def _method(strategy) -> str:
         return "fp32"
     if s == SimusStrategy.HIP_MFMA:
         return "mfma"
-    raise NotImplementedError(f"strategy {s.value!r} is a host strategy of the reference; this build runs HIP only")
+    raise NotImplementedError(f"strategy {s.value!r} is a host strategy of the reference; this package runs the HIP kernels only")
 
 
 def _ndim(a) -> int:
