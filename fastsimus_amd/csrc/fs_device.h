@@ -85,10 +85,21 @@ struct GeoF {
   float sint;  // sin(theta)
 };
 
-__device__ __forceinline__ GeoF geometry_fast(const Problem& p, double xs, double zs, int e, int j) {
+// The element-side constants of geometry_fast for (element e, sub-element j):
+// centroid and element orientation. A lane that keeps its element over a loop
+// of scatterers loads them once.
+struct SubGeo {
+  double x, z;
+  float ce, se;
+};
+__device__ __forceinline__ SubGeo sub_geo(const Problem& p, int e, int j) {
   const int idx = e * p.NS + j;
-  const double dx = xs - p.sub_x[idx];
-  const double dz = zs - p.sub_z[idx];
+  return {p.sub_x[idx], p.sub_z[idx], (float)p.ecos[e], (float)p.esin[e]};
+}
+
+__device__ __forceinline__ GeoF geometry_fast(const Problem& p, double xs, double zs, const SubGeo& sg) {
+  const double dx = xs - sg.x;
+  const double dz = zs - sg.z;
   const double r2 = fma(dx, dx, dz * dz);
   const bool zero = !(r2 > 0.0);
   double y = __builtin_amdgcn_rsq(zero ? 1.0 : r2);
@@ -98,9 +109,8 @@ __device__ __forceinline__ GeoF geometry_fast(const Problem& p, double xs, doubl
   const float rinv = (float)y;
   const float sa = zero ? 1.0f : (float)dx * rinv;
   const float ca = zero ? 0.0f : fabsf((float)dz) * rinv;
-  const float ce = (float)p.ecos[e], se = (float)p.esin[e];
-  const float st = sa * ce - ca * se;
-  const float ct = ca * ce + sa * se;
+  const float st = sa * sg.ce - ca * sg.se;
+  const float ct = ca * sg.ce + sa * sg.se;
   float obl = (p.baffle == FS_BAFFLE_SOFT) ? ct : (p.baffle == FS_BAFFLE_RIGID) ? 1.0f : ct / (ct + p.baffle_b);
   if (!(ct > 0.0f)) obl = 1e-16f;
   const float rcf = (float)rcl;
@@ -112,6 +122,10 @@ __device__ __forceinline__ GeoF geometry_fast(const Problem& p, double xs, doubl
   g.att = p.att_coef * rcf * kLog2e;
   g.sint = st;
   return g;
+}
+
+__device__ __forceinline__ GeoF geometry_fast(const Problem& p, double xs, double zs, int e, int j) {
+  return geometry_fast(p, xs, zs, sub_geo(p, e, j));
 }
 
 // amp exp(-att f) exp(i 2 pi frac(f t)) with hardware sin/cos/exp2

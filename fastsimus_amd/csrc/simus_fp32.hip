@@ -144,7 +144,9 @@ template <int NSUB, bool FULLDIR>
 __global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_per_wave, const cf* __restrict__ Q,
                                             cf* __restrict__ part) {
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  // wave-uniform by construction; readfirstlane tells the compiler, so that the
+  // scatterer loop is a scalar loop and Q, x, z, rc are scalar loads
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int e = blockIdx.x * 64 + lane;
   const int fb = fb0 + blockIdx.y;
   const int k0 = fb * FB;
@@ -160,6 +162,9 @@ __global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_
   const int sb = (chunk * 4 + wave) * s_per_wave;
   const int se = min(sb + s_per_wave, p.S);
   const cf* Qb = Q + (size_t)blockIdx.y * p.S * FB;
+  SubGeo sg[NSUB];
+#pragma unroll
+  for (int j = 0; j < NSUB; ++j) sg[j] = sub_geo(p, ee, j);
   for (int s = sb; s < se; ++s) {
     const cf* Qs = Qb + (size_t)s * FB;
     if (p.rc[s] == 0.f) continue;  // wave-uniform: the scatterer contributes nothing
@@ -169,7 +174,7 @@ __global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_
     GeoF geo[NSUB];
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
-      geo[j] = geometry_fast(p, xs, zs, ee, j);
+      geo[j] = geometry_fast(p, xs, zs, sg[j]);
       g[j] = green_fast(geo[j], fstart, geo[j].amp);
       q[j] = green_step_fast(geo[j], p.df);
     }
