@@ -170,6 +170,11 @@ __global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_
     if (p.rc[s] == 0.f) continue;  // wave-uniform: the scatterer contributes nothing
     const float xs = p.x[s], zs = p.z[s];
     if (out_of_field(p, xs, zs)) continue;
+    // the scatterer's 32 Q values (scalar loads) are requested before the geometry,
+    // whose vector work then covers their latency
+    cf qv[FB];
+#pragma unroll
+    for (int k = 0; k < FB; ++k) qv[k] = Qs[k];
     cf g[NSUB], q[NSUB];
     GeoF geo[NSUB];
 #pragma unroll
@@ -193,7 +198,7 @@ __global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_
         }
         g[j] = cmul(g[j], q[j]);
       }
-      cmac(X[k], G, Qs[k]);
+      cmac(X[k], G, qv[k]);
     }
   }
   if (!valid) return;
