@@ -322,6 +322,7 @@ def bench_single(args) -> int:
     from threadpoolctl import threadpool_limits
 
     import fastsimus_amd as fs
+    from fastsimus_amd import _lib
     from fastsimus_amd import kernels as K
     from oracle import simus_oracle as O
 
@@ -334,18 +335,27 @@ def bench_single(args) -> int:
                          freqs=plan.selected_freqs, pulse=plan.pulse_spectrum, probe=plan.probe_spectrum,
                          correction=plan.correction_factor, full_frequency_directivity=False, device=dev)
 
+    lib = _lib.load()
+
     def timed(prob, plan, method):
+        """Seconds per step, transmit 0's RF, and ms per step of the dominant kernels (k_tx + k_rx for
+        fp32, k_msm for mfma) by HIP events on the stream they run on."""
         def step():
             sp = K.run_spectrum(prob, method)
             return K.run_rf(sp, plan.n_freq_full, plan.freq_idx_start, plan.n_fft, want_full=False)[0]
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
+        lib.fs_set_kernel_timing(1)
+        _lib.kernel_time_ms()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             rf = step()
         torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / args.steps, rf[0].cpu().numpy()
+        dt = (time.perf_counter() - t0) / args.steps
+        k_ms, _ = _lib.kernel_time_ms()
+        lib.fs_set_kernel_timing(0)
+        return dt, rf[0].cpu().numpy(), k_ms / args.steps
 
     out = {}
     # 2 000 scatterers: both device methods, FP64 oracle in full on one core
@@ -363,7 +373,7 @@ def bench_single(args) -> int:
          "cpu_baseline": {"value": evals / t_cpu / 1e9, "unit": UNIT, "cores": 1, "kind": "port",
                           "sample": f"all {x.size} scatterers, FP64 NumPy, {t_cpu:.2f} s"}}
     for method in ("fp32", "mfma"):
-        t, rf = timed(prob, plan, method)
+        t, rf, _ = timed(prob, plan, method)
         r[method] = {"value": evals / t / 1e9, "ms_per_step": t * 1e3, "rf_rel_l2_vs_fp64": rel_l2(rf, want)}
     out["2000_scatterers_zero_delays"] = r
     # 50 010 scatterers: plain FP32 kernels; FP64 error on a sample of the same plan
@@ -373,8 +383,8 @@ def bench_single(args) -> int:
     evals = x.size * params.n_elements * plan.selected_freqs.shape[0]
     r = {"n_scat": int(x.size), "n_freq": int(plan.selected_freqs.shape[0])}
     for method in ("fp32", "mfma"):
-        t, _ = timed(problem(x, z, rc, d, plan), plan, method)
-        r[method] = {"value": evals / t / 1e9, "ms_per_step": t * 1e3}
+        t, _, k_ms = timed(problem(x, z, rc, d, plan), plan, method)
+        r[method] = {"value": evals / t / 1e9, "ms_per_step": t * 1e3, "kernel_ms": k_ms}
     idx = np.concatenate([np.sort(np.random.default_rng(3).choice(50_000, 1000, replace=False)),
                           np.arange(50_000, 50_010)])
     xs, zs, rs = x[idx], z[idx], rc[idx]
@@ -384,7 +394,7 @@ def bench_single(args) -> int:
                                     medium, plan_o)
         t_cpu = time.perf_counter() - t0
     for method in ("fp32", "mfma"):
-        _, rf = timed(problem(xs, zs, rs, d, plan), plan, method)
+        _, rf, _ = timed(problem(xs, zs, rs, d, plan), plan, method)
         r[method]["rf_rel_l2_vs_fp64"] = rel_l2(rf, want)
     ev_s = idx.size * params.n_elements * plan_o["freqs"].size
     r["fp64_sample"] = "1 010 scatterers (1 000 speckle + the 10 points), same plan"
@@ -392,6 +402,16 @@ def bench_single(args) -> int:
                          "sample": f"the 1 010-scatterer sample, FP64 NumPy, {t_cpu:.2f} s"}
     out["50010_scatterers_plane_wave"] = r
     main_r = out["50010_scatterers_plane_wave"]["fp32"]
+    # dominant kernels of the value: k_tx + k_rx; 8 n_sub + 16 flops per evaluation, and the VALU
+    # instructions they issue for it (counted with SQ_INSTS_VALU at n_sub = 2)
+    k_s = main_r["kernel_ms"] / 1e3
+    fl = evals * (8 * plan.n_sub + 16)
+    roof = {"bound": "valu", "achieved": fl / k_s / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+            "frac": fl / k_s / PEAK_FP32, "traffic": None, "kernel": "k_tx + k_rx (FP32 VALU)",
+            "kernel_ms": main_r["kernel_ms"],
+            "valu_instructions_per_eval": 41.3 if plan.n_sub == 2 else None,
+            "valu_issue_tlanes_per_s": 41.3 * evals / k_s / 1e12 if plan.n_sub == 2 else None,
+            "valu_issue_peak_tlanes_per_s": PEAK_FP32 / 2 / 1e12}
     res = {
         "metric": "G(scatterer·element·freq)/s, single transmit (plain FP32 and split-BF16 MFMA)",
         "value": main_r["value"], "unit": UNIT, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -400,7 +420,7 @@ def bench_single(args) -> int:
         "config": {"workload": "single transmit, 128-element 7.6 MHz linear array, 0.5 dB/cm/MHz: 2 000 speckle "
                                "scatterers with zero delays, and 50 000 speckle + 10 bright points under a 0 degree "
                                "plane wave (value: the latter, plain FP32)", "n_elem": params.n_elements},
-        "configs": out, "roofline": None,
+        "configs": out, "roofline": roof,
         "cpu_baseline": out["50010_scatterers_plane_wave"]["cpu_baseline"],
     }
     print(json.dumps(res), flush=True)
@@ -595,12 +615,15 @@ def main() -> int:
              "roofline": {"bound": "valu", "achieved": fl_plain / (kp_ms / 1e3) / 1e12, "peak": PEAK_FP32 / 1e12,
                           "unit": "TFLOP/s", "frac": fl_plain / (kp_ms / 1e3) / PEAK_FP32,
                           "kernel": "k_tx + k_rx (FP32 VALU)", "kernel_ms_total": kp_ms, "launches": kp_n,
-                          # the peak counts packed FP32 FMA, which the kernels do not use: the plain
-                          # kernels issue ~34.5 VALU instructions per evaluation (n_sub = 2), so their
-                          # bound is the VALU issue rate of 256 CUs x 64 lanes per cycle
-                          "valu_instructions_per_eval": 34.5 if plan.n_sub == 2 else None,
-                          "valu_issue_tlanes_per_s": (34.5 * evals_p / (kp_ms / 1e3) / 1e12
-                                                      if plan.n_sub == 2 else None)}}
+                          # 32 of the flops per evaluation are useful, but the kernels issue 41.3 VALU
+                          # instructions per evaluation at n_sub = 2 (k_tx 20.6 + k_rx 20.7, counted
+                          # with SQ_INSTS_VALU), so their bound is the VALU issue rate: 256 CUs x 128
+                          # lanes per cycle (two wave64 instructions every 4 cycles per SIMD, reached
+                          # without packed FP32) = 78.6 T lane-instructions/s at 2.4 GHz
+                          "valu_instructions_per_eval": 41.3 if plan.n_sub == 2 else None,
+                          "valu_issue_tlanes_per_s": (41.3 * evals_p / (kp_ms / 1e3) / 1e12
+                                                      if plan.n_sub == 2 else None),
+                          "valu_issue_peak_tlanes_per_s": PEAK_FP32 / 2 / 1e12}}
     err = {"mfma_vs_fp32_full": rel_l2(rf_m[sub].cpu().numpy(), rf_p.cpu().numpy())}
 
     # FP64 oracle on a scatterer subsample: CPU baseline and absolute error of both GPU paths
