@@ -12,9 +12,9 @@
 //    each (s, e, j) term is reseeded from an FP64 phase f * r_c / c mod 1 and
 //    advanced by an FP32 complex recurrence inside the block;
 //  * k_tx: lane = scatterer, elements are walked uniformly by the wave, so the
-//    transmit table A(f, e) is a wave-uniform (scalar) load; the 32 P(f, s)
-//    accumulators live in VGPRs; the result Q = rc PP P goes to HBM as
-//    Q[block][s][32];
+//    transmit table A(f, e) is a wave-uniform (scalar) load; a thread takes one
+//    or two frequency blocks, whose 32 P(f, s) accumulators each live in VGPRs;
+//    the result Q = rc PP P goes to HBM as Q[block][s][32];
 //  * k_rx: lane = element, scatterers are walked uniformly, so Q(f, s) is a
 //    scalar load; the 32 X(f, e) accumulators live in VGPRs; every wave writes a
 //    private partial sum (no atomics) and k_reduce adds them in a fixed order,
@@ -63,70 +63,86 @@ __global__ void k_spectra(Problem p, int nfp, cf* __restrict__ pp, float* __rest
   }
 }
 
-// MODE 0 (simus): Q[blockIdx.y][s][k] = rc_s PP_k P_k(s).
+// A thread takes BPT consecutive 32-frequency blocks of the launch (lb = BPT blockIdx.y + b,
+// b < BPT, lb < nb), so that the per-(scatterer, element, sub-element) geometry and frequency
+// step are evaluated once for all of them; every block is still seeded from its own FP64 phase.
+// MODE 0 (simus): Q[lb][s][k] = rc_s PP_k P_k(s).
 // MODE 1 (pfield): pw[fb][s] = sum_{k in block} |PP_k P_k(s)|^2 (reference _freq_step_body, _pfield_strategies.py:21-50).
-template <int NSUB, bool FULLDIR, int MODE>
-__global__ __launch_bounds__(256) void k_tx(Problem p, int tx, int nfp, int fb0, const cf* __restrict__ A,
+template <int NSUB, bool FULLDIR, int MODE, int BPT>
+__global__ __launch_bounds__(256) void k_tx(Problem p, int tx, int nfp, int fb0, int nb, const cf* __restrict__ A,
                                             const cf* __restrict__ pp, cf* __restrict__ Q, float* __restrict__ pw) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int fb = fb0 + blockIdx.y;
-  const int k0 = fb * FB;
+  const int lb0 = blockIdx.y * BPT;
   const bool valid = s < p.S;
   const float xs = valid ? p.x[s] : 0.f;
   const float zs = valid ? p.z[s] : 1e-2f;
   const float rcs = (MODE == 0 && valid) ? p.rc[s] : 1.f;
   const bool out = !valid || out_of_field(p, xs, zs) || rcs == 0.f;
 
-  cf P[FB];
+  cf P[BPT][FB];
 #pragma unroll
-  for (int k = 0; k < FB; ++k) P[k] = {0.f, 0.f};
+  for (int b = 0; b < BPT; ++b)
+#pragma unroll
+    for (int k = 0; k < FB; ++k) P[b][k] = {0.f, 0.f};
 
-  const double fstart = p.f0 + (double)k0 * p.df;
-  const cf* At = A + (size_t)tx * p.E * nfp + k0;
+  const cf* At = A + (size_t)tx * p.E * nfp + (size_t)(fb0 + lb0) * FB;
   for (int e = 0; e < p.E; ++e) {
-    cf g[NSUB], q[NSUB];
     GeoF geo[NSUB];
+    cf q[NSUB];
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
       geo[j] = geometry_fast(p, xs, zs, e, j);
-      g[j] = green_fast(geo[j], fstart, geo[j].amp);
       q[j] = green_step_fast(geo[j], p.df);
     }
-    const cf* Ae = At + (size_t)e * nfp;
 #pragma unroll
-    for (int k = 0; k < FB; ++k) {
-      cf G = {0.f, 0.f};
+    for (int b = 0; b < BPT; ++b) {
+      if (lb0 + b >= nb) break;  // uniform over the launch row
+      const double fstart = p.f0 + (double)((fb0 + lb0 + b) * FB) * p.df;
+      cf g[NSUB];
 #pragma unroll
-      for (int j = 0; j < NSUB; ++j) {
-        if constexpr (FULLDIR) {
-          const float d = sinc_fast(p.kappa_per_hz * (float)(fstart + (double)k * p.df) * geo[j].sint);
-          G.x = fmaf(g[j].x, d, G.x);
-          G.y = fmaf(g[j].y, d, G.y);
-        } else {
-          G.x += g[j].x;
-          G.y += g[j].y;
+      for (int j = 0; j < NSUB; ++j) g[j] = green_fast(geo[j], fstart, geo[j].amp);
+      const cf* Ae = At + (size_t)e * nfp + b * FB;
+#pragma unroll
+      for (int k = 0; k < FB; ++k) {
+        cf G = {0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < NSUB; ++j) {
+          if constexpr (FULLDIR) {
+            const float d = sinc_fast(p.kappa_per_hz * (float)(fstart + (double)k * p.df) * geo[j].sint);
+            G.x = fmaf(g[j].x, d, G.x);
+            G.y = fmaf(g[j].y, d, G.y);
+          } else {
+            G.x += g[j].x;
+            G.y += g[j].y;
+          }
+          g[j] = cmul(g[j], q[j]);
         }
-        g[j] = cmul(g[j], q[j]);
+        cmac(P[b][k], G, Ae[k]);
       }
-      cmac(P[k], G, Ae[k]);
     }
   }
   if (!valid) return;
-  if constexpr (MODE == 0) {
-    cf* Qs = Q + ((size_t)blockIdx.y * p.S + s) * FB;
 #pragma unroll
-    for (int k = 0; k < FB; ++k) {
-      cf v = out ? cf{0.f, 0.f} : cmul(P[k], pp[k0 + k]);
-      Qs[k] = {v.x * rcs, v.y * rcs};
-    }
-  } else {
-    float acc = 0.f;
+  for (int b = 0; b < BPT; ++b) {
+    const int lb = lb0 + b;
+    if (lb >= nb) break;
+    const int k0 = (fb0 + lb) * FB;
+    if constexpr (MODE == 0) {
+      cf* Qs = Q + ((size_t)lb * p.S + s) * FB;
 #pragma unroll
-    for (int k = 0; k < FB; ++k) {
-      const cf v = cmul(P[k], pp[k0 + k]);  // pp is zero beyond NF
-      acc = fmaf(v.x, v.x, fmaf(v.y, v.y, acc));
+      for (int k = 0; k < FB; ++k) {
+        cf v = out ? cf{0.f, 0.f} : cmul(P[b][k], pp[k0 + k]);
+        Qs[k] = {v.x * rcs, v.y * rcs};
+      }
+    } else {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < FB; ++k) {
+        const cf v = cmul(P[b][k], pp[k0 + k]);  // pp is zero beyond NF
+        acc = fmaf(v.x, v.x, fmaf(v.y, v.y, acc));
+      }
+      pw[(size_t)(fb0 + lb) * p.S + s] = out ? 0.f : acc;
     }
-    pw[(size_t)fb * p.S + s] = out ? 0.f : acc;
   }
 }
 
@@ -278,12 +294,28 @@ Fp32Plan fp32_plan(const Problem& p) {
   return P;
 }
 
+// Two 32-frequency blocks per k_tx thread share its geometry but halve the grid and take about
+// 64 more VGPRs (3 instead of 5 waves per SIMD): worth it where the halved grid still gives about
+// 8 workgroups per CU of a 256-CU device (50 000 scatterers x 55 blocks: 3 % faster; 2 000
+// scatterers x 52 blocks: 33 % slower, so one block there).
+constexpr long kTwoBlockMinGroups = 2048;
+static bool two_blocks_per_thread(int sx, int nb) { return (long)sx * ((nb + 1) / 2) >= kTwoBlockMinGroups; }
+
 template <int NSUB, bool FD>
 static void launch_tx(const Problem& p, int tx, const Fp32Plan& P, int fb0, int nb, const cf* A, const cf* pp, cf* Q,
                       hipStream_t st) {
-  dim3 grid((p.S + 255) / 256, nb);
+  const int sx = (p.S + 255) / 256;
   const int slot = timing_start(st);
-  hipLaunchKernelGGL((k_tx<NSUB, FD, 0>), grid, dim3(256), 0, st, p, tx, P.nfp, fb0, A, pp, Q, (float*)nullptr);
+  if constexpr (!FD) {
+    if (two_blocks_per_thread(sx, nb)) {
+      hipLaunchKernelGGL((k_tx<NSUB, false, 0, 2>), dim3(sx, (nb + 1) / 2), dim3(256), 0, st, p, tx, P.nfp, fb0, nb, A,
+                         pp, Q, (float*)nullptr);
+      timing_stop(slot, st);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_tx<NSUB, FD, 0, 1>), dim3(sx, nb), dim3(256), 0, st, p, tx, P.nfp, fb0, nb, A, pp, Q,
+                     (float*)nullptr);
   timing_stop(slot, st);
 }
 template <int NSUB, bool FD>
@@ -366,11 +398,16 @@ size_t pfield_workspace(const Problem& p) { return pfield_layout(p).total; }
 template <int NSUB>
 static void launch_pfield(const Problem& p, const PfieldLayout& L, const cf* A, const cf* pp, float* pw,
                           hipStream_t st) {
-  dim3 grid((p.S + 255) / 256, L.nfb);
+  const int sx = (p.S + 255) / 256;
   if (p.fulldir)
-    hipLaunchKernelGGL((k_tx<NSUB, true, 1>), grid, dim3(256), 0, st, p, 0, L.nfp, 0, A, pp, (cf*)nullptr, pw);
+    hipLaunchKernelGGL((k_tx<NSUB, true, 1, 1>), dim3(sx, L.nfb), dim3(256), 0, st, p, 0, L.nfp, 0, L.nfb, A, pp,
+                       (cf*)nullptr, pw);
+  else if (two_blocks_per_thread(sx, L.nfb))
+    hipLaunchKernelGGL((k_tx<NSUB, false, 1, 2>), dim3(sx, (L.nfb + 1) / 2), dim3(256), 0, st, p, 0, L.nfp, 0, L.nfb,
+                       A, pp, (cf*)nullptr, pw);
   else
-    hipLaunchKernelGGL((k_tx<NSUB, false, 1>), grid, dim3(256), 0, st, p, 0, L.nfp, 0, A, pp, (cf*)nullptr, pw);
+    hipLaunchKernelGGL((k_tx<NSUB, false, 1, 1>), dim3(sx, L.nfb), dim3(256), 0, st, p, 0, L.nfp, 0, L.nfb, A, pp,
+                       (cf*)nullptr, pw);
 }
 
 int pfield_fp32(const Problem& p, float* out, char* ws, hipStream_t st) {

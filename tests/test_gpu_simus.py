@@ -254,6 +254,29 @@ def test_pfield_matches_oracle(cuda):
     assert rel_l2(pf, want) < FP32_TOL
 
 
+def test_pfield_large_grid_matches_subset_and_oracle(cuda):
+    """A grid large enough for the TX kernel to take two 32-frequency blocks per thread (at least
+    2048 workgroups): 200 of its points against the same points computed alone under the same plan
+    (one block per thread, the same per-point arithmetic), and 16 of them against the FP64 oracle."""
+    p = fs.L11_5v()
+    m = fs.MediumParams(attenuation=0.5)
+    g = f32(np.stack(np.meshgrid(np.linspace(-0.02, 0.02, 300), np.linspace(0.002, 0.05, 256), indexing="ij"), -1))
+    d = fs.focused(_pos(p), np.array([0.0, 0.02]), speed_of_sound=1540.0)
+    plan = fs.pfield_precompute(g, d, p, m)
+    n_blocks = -(-plan.selected_freqs.shape[0] // 32)
+    assert -(-g[..., 0].size // 256) * -(-n_blocks // 2) >= 2048
+    full = fs.pfield_compute(g, d, plan, p, m).cpu().numpy().ravel()
+    pts = g.reshape(-1, 2)
+    idx = np.sort(np.random.default_rng(11).choice(full.size, 200, replace=False))
+    sub = fs.pfield_compute(pts[idx], d, plan, p, m).cpu().numpy()
+    np.testing.assert_allclose(full[idx], sub, rtol=1e-6, atol=0)
+    pl = O.pfield_plan(pts[:, 0].astype(np.float64), pts[:, 1].astype(np.float64), d, p, m)
+    assert pl["freqs"].size == plan.selected_freqs.shape[0]
+    o = idx[:16]
+    want = O.pfield_oracle(pts[o, 0].astype(np.float64), pts[o, 1].astype(np.float64), d, p, m, pl)
+    assert rel_l2(full[o], want) < FP32_TOL
+
+
 def test_pfield_convex_out_of_field(cuda):
     p = fs.C5_2v()
     g = f32(np.array([[0.0, 0.03], [0.0, 0.001], [0.01, 0.04], [0.0, -0.01]]))
