@@ -38,6 +38,9 @@ METRIC = "G(scatterer·element·freq)/s, batched MFMA vs plain FP32 at ≤1e-3 r
 UNIT = "G(scatterer·element·freq)/s"
 PEAK_BF16 = 2.5e15    # MI355X dense BF16 MFMA, FLOP/s
 PEAK_FP32 = 157.3e12  # MI355X FP32 vector, FLOP/s
+# VALU instructions the plain kernels (k_tx + k_rx) issue per evaluation at n_sub = 2 on the
+# benchmarked sizes (SQ_INSTS_VALU x 64 / evaluations: k_tx 18.7, k_rx 20.7)
+VALU_PER_EVAL_NSUB2 = 39.4
 
 
 def workload(seed: int = 2026, n_scat: int = 200_000, n_tx: int = 75):
@@ -409,8 +412,8 @@ def bench_single(args) -> int:
     roof = {"bound": "valu", "achieved": fl / k_s / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
             "frac": fl / k_s / PEAK_FP32, "traffic": None, "kernel": "k_tx + k_rx (FP32 VALU)",
             "kernel_ms": main_r["kernel_ms"],
-            "valu_instructions_per_eval": 41.3 if plan.n_sub == 2 else None,
-            "valu_issue_tlanes_per_s": 41.3 * evals / k_s / 1e12 if plan.n_sub == 2 else None,
+            "valu_instructions_per_eval": VALU_PER_EVAL_NSUB2 if plan.n_sub == 2 else None,
+            "valu_issue_tlanes_per_s": VALU_PER_EVAL_NSUB2 * evals / k_s / 1e12 if plan.n_sub == 2 else None,
             "valu_issue_peak_tlanes_per_s": PEAK_FP32 / 2 / 1e12}
     res = {
         "metric": "G(scatterer·element·freq)/s, single transmit (plain FP32 and split-BF16 MFMA)",
@@ -615,13 +618,14 @@ def main() -> int:
              "roofline": {"bound": "valu", "achieved": fl_plain / (kp_ms / 1e3) / 1e12, "peak": PEAK_FP32 / 1e12,
                           "unit": "TFLOP/s", "frac": fl_plain / (kp_ms / 1e3) / PEAK_FP32,
                           "kernel": "k_tx + k_rx (FP32 VALU)", "kernel_ms_total": kp_ms, "launches": kp_n,
-                          # 32 of the flops per evaluation are useful, but the kernels issue 41.3 VALU
-                          # instructions per evaluation at n_sub = 2 (k_tx 20.6 + k_rx 20.7, counted
-                          # with SQ_INSTS_VALU), so their bound is the VALU issue rate: 256 CUs x 128
-                          # lanes per cycle (two wave64 instructions every 4 cycles per SIMD, reached
-                          # without packed FP32) = 78.6 T lane-instructions/s at 2.4 GHz
-                          "valu_instructions_per_eval": 41.3 if plan.n_sub == 2 else None,
-                          "valu_issue_tlanes_per_s": (41.3 * evals_p / (kp_ms / 1e3) / 1e12
+                          # 32 of the flops per evaluation are useful, but the kernels issue 39.4 VALU
+                          # instructions per evaluation at n_sub = 2 (k_tx 18.7 with two frequency
+                          # blocks per thread + k_rx 20.7, counted with SQ_INSTS_VALU), so their bound
+                          # is the VALU issue rate: 256 CUs x 128 lanes per cycle (two wave64
+                          # instructions every 4 cycles per SIMD, reached without packed FP32) =
+                          # 78.6 T lane-instructions/s at 2.4 GHz
+                          "valu_instructions_per_eval": VALU_PER_EVAL_NSUB2 if plan.n_sub == 2 else None,
+                          "valu_issue_tlanes_per_s": (VALU_PER_EVAL_NSUB2 * evals_p / (kp_ms / 1e3) / 1e12
                                                       if plan.n_sub == 2 else None),
                           "valu_issue_peak_tlanes_per_s": PEAK_FP32 / 2 / 1e12}}
     err = {"mfma_vs_fp32_full": rel_l2(rf_m[sub].cpu().numpy(), rf_p.cpu().numpy())}
