@@ -241,6 +241,30 @@ def test_full_size_linearity_and_symmetry(cuda):
     assert float(torch.linalg.norm(mm - r32) / torch.linalg.norm(r32)) < MFMA_TOL
 
 
+@pytest.mark.parametrize("n_sub", [1, 3])
+def test_two_block_tx_path_other_sub_element_counts(n_sub, cuda):
+    """40 000 scatterers x 52 frequency blocks: the TX kernel takes two 32-frequency blocks per thread
+    (at least 2048 workgroups). The plain spectrum against the split-BF16 kernel, which shares no code
+    with it, and superposition over two halves of the reflectivity."""
+    import torch
+
+    p, m = fs.L11_5v(), fs.MediumParams(attenuation=0.3)
+    x, z, rc = speckle(21, 40000)
+    sc = np.stack([x, z], -1)
+    plan = fs.simus_precompute(sc, rc, np.zeros(128), p, m, element_splitting=n_sub)
+    assert plan.n_sub == n_sub
+    n_blocks = -(-plan.selected_freqs.shape[0] // 32)
+    assert -(-x.size // 256) * -(-n_blocks // 2) >= 2048
+    r32 = fs.simus_compute(sc, rc, np.zeros(128), plan, p, m)
+    rmm = fs.simus_compute(sc, rc, np.zeros(128), plan, p, m, strategy="hip_mfma")
+    assert float(torch.linalg.norm(rmm.rf - r32.rf) / torch.linalg.norm(r32.rf)) < MFMA_TOL
+    half = rc.copy()
+    half[1::2] = 0.0
+    s_a = fs.simus_compute(sc, half, np.zeros(128), plan, p, m).spectrum
+    s_b = fs.simus_compute(sc, rc - half, np.zeros(128), plan, p, m).spectrum
+    assert float(torch.linalg.norm(s_a + s_b - r32.spectrum) / torch.linalg.norm(r32.spectrum)) < 1e-5
+
+
 # ----------------------------------------------------------------- pfield
 def test_pfield_matches_oracle(cuda):
     p = fs.L11_5v()
