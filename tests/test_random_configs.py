@@ -18,7 +18,7 @@ import fastsimus_amd as fs
 from oracle import simus_oracle as O
 from tests.conftest import f32, rel_l2
 
-SEEDS = list(range(48))
+SEEDS = list(range(448))
 
 
 def random_case(seed: int):
@@ -96,7 +96,7 @@ def test_random_configuration_matches_oracle(seed, cuda):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("seed", list(range(24)))
+@pytest.mark.parametrize("seed", list(range(224)))
 def test_random_pfield_matches_oracle(seed, cuda):
     """pfield on random arrays, media and grids (points behind the array included) against the FP64 oracle."""
     params, medium, _, _, _, D, kw = random_case(seed)
@@ -114,8 +114,10 @@ def test_random_pfield_matches_oracle(seed, cuda):
     assert e < 1e-4
 
 
-# the first 8 draws from seed 100 on whose array the split-BF16 method takes (at most 128 elements)
-MEDIUM_SEEDS = [s for s in range(100, 200) if random_case(s)[0].n_elements <= 128][:8]
+# draws whose array the split-BF16 method takes (at most 128 elements): the first 8 from seed 100 on
+# and the first 40 from seed 200 on
+MEDIUM_SEEDS = ([s for s in range(100, 200) if random_case(s)[0].n_elements <= 128][:8]
+                + [s for s in range(200, 1000) if random_case(s)[0].n_elements <= 128][:40])
 
 
 @pytest.mark.gpu
