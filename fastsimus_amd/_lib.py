@@ -16,6 +16,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libfastsimus.so"
 
 FS_OK, FS_ERR_INVALID, FS_ERR_UNSUPPORTED, FS_ERR_WORKSPACE, FS_ERR_LAUNCH, FS_ERR_RESOURCE = range(6)
 FS_METHOD_FP32, FS_METHOD_MFMA_BF16X3 = 0, 1
+FS_ADJOINT_AUTO, FS_ADJOINT_MATRIX, FS_ADJOINT_VECTOR = 0, 1, 2
 
 # every symbol declared in include/fastsimus.h
 EXPORTS = (
@@ -33,7 +34,6 @@ EXPORTS = (
     "fs_rf_adjoint",
     "fs_simus_adjoint_workspace_bytes",
     "fs_simus_adjoint",
-    "fs_set_adjoint_method",
     "fs_pfield_workspace_bytes",
     "fs_pfield",
     "fs_max_distance",
@@ -105,11 +105,9 @@ def load() -> C.CDLL:
     lib.fs_rf_adjoint.restype = C.c_int
     lib.fs_rf_adjoint.argtypes = [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, C.c_size_t, P]
     lib.fs_simus_adjoint_workspace_bytes.restype = C.c_size_t
-    lib.fs_simus_adjoint_workspace_bytes.argtypes = [D]
+    lib.fs_simus_adjoint_workspace_bytes.argtypes = [D, C.c_int]
     lib.fs_simus_adjoint.restype = C.c_int
-    lib.fs_simus_adjoint.argtypes = [D, P, P, P, P, P, P, C.c_size_t, P]
-    lib.fs_set_adjoint_method.restype = C.c_int
-    lib.fs_set_adjoint_method.argtypes = [C.c_int]
+    lib.fs_simus_adjoint.argtypes = [D, C.c_int, P, P, P, P, P, P, C.c_size_t, P]
     lib.fs_pfield_workspace_bytes.restype = C.c_size_t
     lib.fs_pfield_workspace_bytes.argtypes = [D]
     lib.fs_pfield.restype = C.c_int

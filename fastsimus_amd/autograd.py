@@ -36,8 +36,8 @@ class _State:
         self.adjoint = adjoint
 
 
-# fs_set_adjoint_method codes
-_ADJOINT = {"auto": 0, "matrix": 1, "vector": 2}
+# fs_adjoint_method codes, passed with every fs_simus_adjoint call
+_ADJOINT = {"auto": _lib.FS_ADJOINT_AUTO, "matrix": _lib.FS_ADJOINT_MATRIX, "vector": _lib.FS_ADJOINT_VECTOR}
 
 
 # From this many transmits on, the multistatic-matrix (MFMA) forward, whose cost does not grow
@@ -89,13 +89,10 @@ class _SimusRF(torch.autograd.Function):
         d_x = torch.empty(S, dtype=torch.float32, device=dev)
         d_z = torch.empty(S, dtype=torch.float32, device=dev)
         d_c = torch.empty(1, dtype=torch.float64, device=dev)
-        _lib.check(lib.fs_set_adjoint_method(_ADJOINT[st.adjoint]), "fs_set_adjoint_method")
-        try:
-            ws2 = K._workspace(lib.fs_simus_adjoint_workspace_bytes(prob.dref), dev)
-            s = lib.fs_simus_adjoint(prob.dref, d_spect.data_ptr(), d_rc.data_ptr(), d_x.data_ptr(), d_z.data_ptr(),
-                                     d_c.data_ptr(), ws2.data_ptr(), ws2.numel(), stream)
-        finally:
-            lib.fs_set_adjoint_method(0)
+        method = _ADJOINT[st.adjoint]
+        ws2 = K._workspace(lib.fs_simus_adjoint_workspace_bytes(prob.dref, method), dev)
+        s = lib.fs_simus_adjoint(prob.dref, method, d_spect.data_ptr(), d_rc.data_ptr(), d_x.data_ptr(),
+                                 d_z.data_ptr(), d_c.data_ptr(), ws2.data_ptr(), ws2.numel(), stream)
         _lib.check(s, "fs_simus_adjoint")
         dtype, cdev = ctx.c_meta
         g_c = d_c.reshape(()).to(dtype=dtype, device=cdev)

@@ -948,27 +948,22 @@ static int adjoint_matrix(const Problem& p, const cf* dspect, float* d_rc, float
   return hipGetLastError() == hipSuccess ? FS_OK : FS_ERR_LAUNCH;
 }
 
-// 0: matrix cores where they apply (n_elem <= 128, n_sub <= 2), else vector; 1: matrix cores only; 2: vector only.
-static int g_adjoint_method = 0;
-
+// method (fs_adjoint_method, passed with every call): FS_ADJOINT_AUTO = matrix cores where they apply
+// (n_elem <= 128, n_sub <= 2), else vector; FS_ADJOINT_MATRIX = matrix cores only; FS_ADJOINT_VECTOR = vector only.
 static bool matrix_applies(const Problem& p) { return p.E <= 16 * AM_MAXW && p.NS <= AM_NSUB_MAX; }
 
-static bool use_matrix(const Problem& p) { return g_adjoint_method != 2 && matrix_applies(p); }
+static bool use_matrix(const Problem& p, int method) { return method != FS_ADJOINT_VECTOR && matrix_applies(p); }
 
-size_t adjoint_workspace(const Problem& p) { return use_matrix(p) ? adjm_layout(p).total : adj_layout(p).total; }
+size_t adjoint_workspace(const Problem& p, int method) {
+  return use_matrix(p, method) ? adjm_layout(p).total : adj_layout(p).total;
+}
 
-int adjoint_fp32(const Problem& p, const cf* dspect, float* d_rc, float* d_x, float* d_z, double* d_c, char* ws,
-                 hipStream_t st) {
+int adjoint_fp32(const Problem& p, int method, const cf* dspect, float* d_rc, float* d_x, float* d_z, double* d_c,
+                 char* ws, hipStream_t st) {
   if (p.S == 0) return FS_ERR_INVALID;
-  if (g_adjoint_method == 1 && !matrix_applies(p)) return FS_ERR_UNSUPPORTED;
-  if (use_matrix(p)) return adjoint_matrix(p, dspect, d_rc, d_x, d_z, d_c, ws, st);
+  if (method == FS_ADJOINT_MATRIX && !matrix_applies(p)) return FS_ERR_UNSUPPORTED;
+  if (use_matrix(p, method)) return adjoint_matrix(p, dspect, d_rc, d_x, d_z, d_c, ws, st);
   return adjoint_vector(p, dspect, d_rc, d_x, d_z, d_c, ws, st);
 }
 
 }  // namespace fs
-
-extern "C" int fs_set_adjoint_method(int method) {
-  if (method < 0 || method > 2) return FS_ERR_INVALID;
-  fs::g_adjoint_method = method;
-  return FS_OK;
-}

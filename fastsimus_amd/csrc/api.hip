@@ -23,9 +23,9 @@ int spectrum_to_rf(const cf* spect, int NT, int NF, int E, int nff, int idx0, in
 size_t rf_adjoint_ws(int NT, int nkeep, int E);
 int rf_adjoint(const float* u, const float* dy, int NT, int NF, int E, int idx0, int N, cf* dspect, char* ws,
                hipStream_t st);
-int adjoint_fp32(const Problem& p, const cf* dspect, float* d_rc, float* d_x, float* d_z, double* d_c, char* ws,
-                 hipStream_t st);
-size_t adjoint_workspace(const Problem& p);
+int adjoint_fp32(const Problem& p, int method, const cf* dspect, float* d_rc, float* d_x, float* d_z, double* d_c,
+                 char* ws, hipStream_t st);
+size_t adjoint_workspace(const Problem& p, int method);
 int pfield_fp32(const Problem& p, float* out, char* ws, hipStream_t st);
 size_t pfield_workspace(const Problem& p);
 int fill_zero(void* ptr, size_t n_words, hipStream_t st);
@@ -235,21 +235,25 @@ int fs_rf_adjoint(const float* rf_ungated, const float* d_rf, int n_tx, int n_fr
                     (hipStream_t)stream);
 }
 
-size_t fs_simus_adjoint_workspace_bytes(const fs_simus_desc* d) {
+static bool valid_adjoint_method(int m) { return m == FS_ADJOINT_AUTO || m == FS_ADJOINT_MATRIX || m == FS_ADJOINT_VECTOR; }
+
+size_t fs_simus_adjoint_workspace_bytes(const fs_simus_desc* d, int adjoint_method) {
   Problem p;
-  if (make_problem(d, &p, true, true) != FS_OK) return 0;
-  return adjoint_workspace(p);
+  if (!valid_adjoint_method(adjoint_method) || make_problem(d, &p, true, true) != FS_OK) return 0;
+  return adjoint_workspace(p, adjoint_method);
 }
 
-int fs_simus_adjoint(const fs_simus_desc* d, const float* d_spect, float* d_rc, float* d_x, float* d_z, double* d_c,
-                     void* workspace, size_t workspace_bytes, void* stream) {
+int fs_simus_adjoint(const fs_simus_desc* d, int adjoint_method, const float* d_spect, float* d_rc, float* d_x,
+                     float* d_z, double* d_c, void* workspace, size_t workspace_bytes, void* stream) {
   Problem p;
   int rc = make_problem(d, &p, true, true);
   if (rc != FS_OK) return rc;
+  if (!valid_adjoint_method(adjoint_method)) return FS_ERR_INVALID;
   if (!d_spect || !d_rc || !d_x || !d_z || !d_c) return FS_ERR_INVALID;
   if (p.fulldir) return FS_ERR_UNSUPPORTED;
-  if (!workspace || workspace_bytes < adjoint_workspace(p)) return FS_ERR_WORKSPACE;
-  return adjoint_fp32(p, (const cf*)d_spect, d_rc, d_x, d_z, d_c, (char*)workspace, (hipStream_t)stream);
+  if (!workspace || workspace_bytes < adjoint_workspace(p, adjoint_method)) return FS_ERR_WORKSPACE;
+  return adjoint_fp32(p, adjoint_method, (const cf*)d_spect, d_rc, d_x, d_z, d_c, (char*)workspace,
+                      (hipStream_t)stream);
 }
 
 size_t fs_pfield_workspace_bytes(const fs_simus_desc* d) {

@@ -18,7 +18,7 @@
  *   fs_simus_adjoint      <- no kernel in the reference; differentiates the
  *                            algebra of _simus_freq_outer_scan
  *                            (src/fast_simus/_simus_strategies.py:152-210);
- *                            fs_set_adjoint_method selects its kernels
+ *                            its fs_adjoint_method argument selects its kernels
  *   fs_rf_adjoint         <- reverse of _irfft_and_threshold (simus.py:336-367)
  *   fs_pfield             <- pfield_metal / _freq_outer_python
  *                            (src/fast_simus/kernels/metal_pfield.py:89-219,
@@ -158,19 +158,21 @@ size_t fs_rf_adjoint_workspace_bytes(int n_tx, int n_keep, int n_elem);
 int fs_rf_adjoint(const float* rf_ungated, const float* d_rf, int n_tx, int n_freq, int n_elem, int idx0, int n_fft,
                   float* d_spect, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Kernels of fs_simus_adjoint, chosen per call (the same value must be passed to the workspace query):
+ * FS_ADJOINT_AUTO = matrix cores where they apply (n_elem <= 128, n_sub <= 2: the element contractions
+ * as split-BF16 MFMA products, Green's functions generated once per group of 16 transmits), else FP32
+ * vector kernels; FS_ADJOINT_MATRIX = matrix cores only (FS_ERR_UNSUPPORTED where they do not apply);
+ * FS_ADJOINT_VECTOR = vector kernels only (Green's functions regenerated per transmit). No counterpart
+ * in the reference. */
+enum fs_adjoint_method { FS_ADJOINT_AUTO = 0, FS_ADJOINT_MATRIX = 1, FS_ADJOINT_VECTOR = 2 };
+
 /* Gradients of L through fs_simus_spectrum (method FP32 algebra) given dL/d(spect) (the corrected
  * selected spectrum, interleaved complex): d_rc, d_x, d_z [S] float32 (overwritten),
- * d_c [1] float64 (overwritten). Green's functions are recomputed, not stored. */
-size_t fs_simus_adjoint_workspace_bytes(const fs_simus_desc* d);
-int fs_simus_adjoint(const fs_simus_desc* d, const float* d_spect, float* d_rc, float* d_x, float* d_z, double* d_c,
-                     void* workspace, size_t workspace_bytes, void* stream);
-/* Kernels of fs_simus_adjoint (and of its workspace query), process-wide: 0 = matrix cores where they
- * apply (n_elem <= 128, n_sub <= 2: the element contractions as split-BF16 MFMA products, Green's
- * functions generated once per group of 16 transmits), else FP32 vector kernels; 1 = matrix cores
- * only (FS_ERR_UNSUPPORTED where they do not apply); 2 = vector kernels only (Green's functions
- * regenerated per transmit). Not thread-safe; set it before the workspace query. No counterpart in
- * the reference. */
-int fs_set_adjoint_method(int method);
+ * d_c [1] float64 (overwritten). Green's functions are recomputed, not stored. The workspace query
+ * returns 0 for an invalid descriptor or method. */
+size_t fs_simus_adjoint_workspace_bytes(const fs_simus_desc* d, int adjoint_method);
+int fs_simus_adjoint(const fs_simus_desc* d, int adjoint_method, const float* d_spect, float* d_rc, float* d_x,
+                     float* d_z, double* d_c, void* workspace, size_t workspace_bytes, void* stream);
 
 /* RMS pressure field of one transmit on n_scat points (scat_x/scat_z reused as the grid; rc unused):
  *   out[p] = sqrt(correction * sum_k |pulse_k probe_k sum_{e,j} g A|^2), 0 where out of field. */

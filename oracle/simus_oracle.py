@@ -22,9 +22,17 @@ Pinning (what checks each function, see ``tests/test_oracle.py``):
   focused at 3 cm with scatterers to 8 cm, ``tests/test_simus.py:355-386``;
   fewer bins at -60 dB than at -100 dB, ``test_simus.py:325-353``;
   n_fft = 2 (n_ff - 1) when fs = 4 fc).
-* ``geometry`` / ``green_terms``: 3-4-5 distance, lambda/2 clip, on-axis angle,
-  obliquity per baffle, horizon floor, |step| = 1 without attenuation
-  (``tests/test_pfield_math.py:101-234``).
+* ``pulse_spectrum``: pulse(wc) = +i exactly for integer cycle counts (fixes
+  the sign of every RF).
+* ``element_layout`` / ``sub_elements`` (convex): adjacent element centres
+  exactly one pitch apart, every centre at the radius from the arc centre,
+  every sub-element centroid on its element's tangent at the offset o_j,
+  oriented along the direction of increasing element angle.
+* ``geometry`` / ``green_terms`` / ``green_step``: 3-4-5 distance,
+  lambda/2 clip, on-axis angle, obliquity per baffle, horizon floor,
+  |step| = 1 without attenuation (``tests/test_pfield_math.py:101-234``).
+* ``pfield_plan``: the frequency step of a hand-computed case (one on-axis
+  point, the outermost sub-element, a known largest delay).
 * ``spectrum_recurrence`` against ``spectrum_direct`` on brute-force inputs,
   and against the single-element single-scatterer closed form.
 * ``rf_from_spectrum``: the inverse-DFT definition, the zero-input case and
@@ -177,6 +185,12 @@ def simus_plan(x, z, delays, params, medium, fs=None, n_w=1.0, db=-60.0, element
 
 
 # --------------------------------------------------------------------------- spectrum
+def green_step(r, fstep, c, alpha):
+    """Per-frequency-step multiplier of every Green's term, ``_pfield_math.py:203-247``:
+    exp((-alpha(df) + i k(df)) r) with alpha(f) = alpha_dB / NEPER_TO_DB * f / 1e6 * 1e2 (Np/m)."""
+    return np.exp((-(alpha / NEPER_TO_DB * fstep / 1e6 * 1e2) + 1j * 2 * pi * fstep / c) * np.asarray(r))
+
+
 def _tx_weights(delays, apod):
     d = np.asarray(delays, dtype=np.float64)
     a = np.ones_like(d) if apod is None else np.asarray(apod, dtype=np.float64) * np.ones_like(d)
@@ -199,7 +213,7 @@ def spectrum_recurrence(x, z, rc, delays, params, medium, plan, apod=None, full_
     k0r = 2 * pi * f0 / c * r
     a0 = alpha / NEPER_TO_DB * f0 / 1e6 * 1e2
     phase = np.exp(-a0 * r + 1j * (k0r - 2 * pi * np.floor(k0r / (2 * pi))))
-    step = np.exp((-(alpha / NEPER_TO_DB * fstep / 1e6 * 1e2) + 1j * 2 * pi * fstep / c) * r)
+    step = green_step(r, fstep, c, alpha)
     phase = phase * g["amp"] * plan["n_sub"]  # amp carries 1/n_sub; the reference divides in mean()
     d, w = _tx_weights(delays, apod)
     da = np.exp(1j * 2 * pi * f0 * d) * w

@@ -41,9 +41,16 @@ def speckle(seed: int, n: int, xlim=(-0.02, 0.02), zlim=(0.005, 0.045)):
 
 
 def rel_l2(a, b) -> float:
-    a = np.asarray(a)
-    b = np.asarray(b)
-    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+    """||a - b|| / ||b|| in float64 (complex128 for complex inputs). An all-zero reference gives 0 when
+    ``a`` is all zero too and inf otherwise, never NaN."""
+    wide = np.complex128 if (np.iscomplexobj(a) or np.iscomplexobj(b)) else np.float64
+    a = np.asarray(a, dtype=wide).ravel()
+    b = np.asarray(b, dtype=wide).ravel()
+    num = float(np.linalg.norm(a - b))
+    den = float(np.linalg.norm(b))
+    if den == 0.0:
+        return 0.0 if num == 0.0 else float("inf")
+    return num / den
 
 
 @pytest.fixture(scope="session")

@@ -178,20 +178,16 @@ def _spectral_adjoint(prob, g, adjoint):
     from fastsimus_amd import kernels as K
 
     lib = _lib.load()
-    code = {"matrix": 1, "vector": 2}[adjoint]
+    code = {"matrix": _lib.FS_ADJOINT_MATRIX, "vector": _lib.FS_ADJOINT_VECTOR}[adjoint]
     S = prob.x.numel()
     # the C ABI reads n_tx * n_freq * n_elem interleaved complex values: check the buffer on the host
     assert tuple(g.shape) == (prob.n_tx, prob.n_freq, prob.params.n_elements, 2)
     assert g.dtype == torch.float32 and g.is_contiguous() and g.device == prob.device
     out = [torch.empty(S, dtype=torch.float32, device=prob.device) for _ in range(3)]
     d_c = torch.empty(1, dtype=torch.float64, device=prob.device)
-    _lib.check(lib.fs_set_adjoint_method(code), "fs_set_adjoint_method")
-    try:
-        ws = K._workspace(lib.fs_simus_adjoint_workspace_bytes(prob.dref), prob.device)
-        st = lib.fs_simus_adjoint(prob.dref, g.data_ptr(), *(t.data_ptr() for t in out), d_c.data_ptr(), ws.data_ptr(),
-                                  ws.numel(), K._stream(prob.device))
-    finally:
-        lib.fs_set_adjoint_method(0)
+    ws = K._workspace(lib.fs_simus_adjoint_workspace_bytes(prob.dref, code), prob.device)
+    st = lib.fs_simus_adjoint(prob.dref, code, g.data_ptr(), *(t.data_ptr() for t in out), d_c.data_ptr(),
+                              ws.data_ptr(), ws.numel(), K._stream(prob.device))
     _lib.check(st, "fs_simus_adjoint")
     torch.cuda.synchronize()
     return [t.cpu().numpy().astype(np.float64) for t in out] + [float(d_c.cpu())]

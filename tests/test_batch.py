@@ -98,8 +98,11 @@ def test_batch_equals_single_phantom_calls(cuda):
             one = fs.simus_compute(sc, rc, D, plan, p, m, strategy=strategy).rf.cpu().numpy()
             if strategy == "hip":
                 np.testing.assert_array_equal(rf[q], one)
+            elif sc.shape[0] == 0:
+                assert not np.any(one) and not np.any(rf[q]), q
             else:
-                assert rel_l2(rf[q], one) < 1e-5 or not np.any(one)
+                assert np.any(one), q
+                assert rel_l2(rf[q], one) < 1e-5, q
 
 
 @pytest.mark.gpu
@@ -126,7 +129,8 @@ def test_batch_above_one_launch_is_split(cuda):
             assert not np.any(rf[q])
             continue
         one = fs.simus_compute(sc, rc, D, plan, p, m, strategy="hip_mfma").rf.cpu().numpy()
-        assert rel_l2(rf[q], one) < 1e-5 or not np.any(one), q
+        assert np.any(one), q
+        assert rel_l2(rf[q], one) < 1e-5, q
 
 
 @pytest.mark.gpu

@@ -205,7 +205,7 @@ def test_library_rejects_bad_descriptors():
     assert lib.fs_simus_spectrum(None, 0, None, None, 0, None) == _lib.FS_ERR_INVALID
     assert lib.fs_spectrum_to_rf(None, 1, 1, 1, 1, 0, 4, None, None, None, None, 0, None) == _lib.FS_ERR_INVALID
     assert lib.fs_pfield(None, None, None, 0, None) == _lib.FS_ERR_INVALID
-    assert lib.fs_simus_adjoint(None, None, None, None, None, None, None, 0, None) == _lib.FS_ERR_INVALID
+    assert lib.fs_simus_adjoint(None, 0, None, None, None, None, None, None, 0, None) == _lib.FS_ERR_INVALID
     assert lib.fs_simus_spectrum_batch(None, 1, 1, None, None, None, 0, None) == _lib.FS_ERR_INVALID
     assert lib.fs_tuning_get(None, 1, None) == _lib.FS_ERR_INVALID
     with pytest.raises(ValueError):
@@ -277,11 +277,29 @@ def test_rf_stack_layout_and_padding():
         fs.rf_stack([])
 
 
-def test_adjoint_method_switch_validates():
-    """fs_set_adjoint_method takes 0 (auto), 1 (matrix cores), 2 (vector kernels); anything else is
-    FS_ERR_INVALID and leaves the choice unchanged. No GPU call is made."""
+def _host_desc(n_scat=10, n_elem=8, n_sub=1, n_freq=40, n_tx=2):
+    """A descriptor whose pointers are non-null placeholders: valid for size queries and argument checks that
+    return before any device work, never for a launch."""
+    d = _lib.SimusDesc()
+    d.n_scat, d.n_elem, d.n_sub, d.n_freq, d.n_tx = n_scat, n_elem, n_sub, n_freq, n_tx
+    for f in ("scat_x", "scat_z", "rc", "sub_x", "sub_z", "elem_cos", "elem_sin", "tx_delay", "tx_apod", "pulse_re",
+              "pulse_im", "probe"):
+        setattr(d, f, 8)
+    d.f0, d.df, d.speed_of_sound, d.freq_center, d.seg_length = 2e6, 1e4, 1540.0, 5e6, 2e-4
+    d.radius, d.correction = inf, 1.0
+    return d
+
+
+def test_adjoint_method_is_a_per_call_argument():
+    """The adjoint kernels are chosen per call (0 auto, 1 matrix cores, 2 vector kernels): the workspace query
+    returns 0 and the call FS_ERR_INVALID for any other value, before any GPU work; there is no process-wide
+    switch left to race on."""
     lib = _lib.load()
-    assert lib.fs_set_adjoint_method(3) == _lib.FS_ERR_INVALID
-    assert lib.fs_set_adjoint_method(-1) == _lib.FS_ERR_INVALID
-    for m in (1, 2, 0):
-        assert lib.fs_set_adjoint_method(m) == 0
+    assert not hasattr(lib, "fs_set_adjoint_method") or "fs_set_adjoint_method" not in _lib.EXPORTS
+    d = _host_desc()
+    for m in (-1, 3):
+        assert lib.fs_simus_adjoint_workspace_bytes(C.byref(d), m) == 0
+        assert lib.fs_simus_adjoint(C.byref(d), m, 1, 1, 1, 1, 1, 1, 1 << 30, None) == _lib.FS_ERR_INVALID
+    sizes = [lib.fs_simus_adjoint_workspace_bytes(C.byref(d), m) for m in (0, 1, 2)]
+    assert all(s > 0 for s in sizes)
+    assert sizes[0] == sizes[1] != sizes[2]  # auto takes the matrix cores at 8 elements, 1 sub-element

@@ -43,6 +43,63 @@ def test_pulse_closed_forms(n_w):
     assert abs(O.pulse_spectrum(np.array([2 * pi * fc]), fc, n_w)[0]) == pytest.approx(1.0, abs=1e-12)
 
 
+@pytest.mark.parametrize("n_w", [1, 2, 3])
+def test_pulse_sign_at_centre(n_w):
+    """pulse(wc) = i (sinc(0) - sinc(2 n_w)) = +i exactly: a flipped sign would negate every RF."""
+    fc = 5e6
+    for fn in (O.pulse_spectrum, fs.spectrum.pulse_spectrum):
+        v = fn(np.array([2 * pi * fc]), fc, n_w)[0]
+        assert abs(v - 1j) < 1e-15, (fn.__module__, v)
+
+
+# ----------------------------------------------------------------- convex geometry (utils/geometry.py:60-84,
+# _pfield_math.py:41-67), oracle and product alike
+def _convex_layouts(p):
+    ex, ez, th, apex = O.element_layout(p)
+    pos, theta, ap = fs.element_positions(p.n_elements, p.pitch, p.radius)
+    return [("oracle", ex, ez, th, apex), ("product", pos[:, 0], pos[:, 1], theta, ap)]
+
+
+@pytest.mark.parametrize("n,pitch,radius", [(128, 3e-4, 49.57e-3), (64, 4.7e-4, 20e-3), (5, 1e-3, 6e-3)])
+def test_convex_elements_one_pitch_apart_on_the_arc(n, pitch, radius):
+    p = fs.TransducerParams(freq_center=3.5e6, pitch=pitch, n_elements=n, kerf=pitch / 8, radius=radius)
+    for who, ex, ez, th, apex in _convex_layouts(p):
+        # adjacent centres are one pitch apart along the chord (chord = 2 R sin(asin(pitch / 2 R)))
+        np.testing.assert_allclose(np.hypot(np.diff(ex), np.diff(ez)), pitch, rtol=1e-12, err_msg=who)
+        # every centre on the circle of the radius around the arc centre (0, -apex), at angle theta
+        np.testing.assert_allclose(np.hypot(ex, ez + apex), radius, rtol=1e-13, err_msg=who)
+        np.testing.assert_allclose(ex, radius * np.sin(th), atol=1e-15, err_msg=who)
+        # symmetric about the axis, chord endpoints at z = 0
+        np.testing.assert_allclose(ex, -ex[::-1], atol=1e-15, err_msg=who)
+        assert abs(ez[0]) < 1e-13 and abs(ez[-1]) < 1e-13, who
+
+
+@pytest.mark.parametrize("n_sub", [1, 2, 3, 5])
+def test_convex_subelements_on_the_tangent(n_sub):
+    """Each sub-element centroid lies on its element's tangent at the offset o_j, oriented along increasing
+    element angle: (sub - centre) . (centre - arc centre) = 0, |sub - centre| = |o_j|, and (sub - centre) has the
+    direction of the chord from the previous to the next element (the tangent of equally spaced points)."""
+    p = fs.C5_2v()
+    E = p.n_elements
+    W = p.element_width
+    seg = W / n_sub
+    o = -W / 2 + seg / 2 + seg * np.arange(n_sub)
+    ex, ez, th, apex = O.element_layout(p)
+    pos, theta, _ = fs.element_positions(E, p.pitch, p.radius)
+    sub_o = np.stack(O.sub_elements(p, n_sub)[:2], -1)                    # (E, n_sub, 2)
+    sx, sz, _, _ = fs.geometry.subelement_positions(p, n_sub)
+    sub_p = np.stack([sx, sz], -1)
+    for who, sub, cx, cz in (("oracle", sub_o, ex, ez), ("product", sub_p, pos[:, 0], pos[:, 1])):
+        d = sub - np.stack([cx, cz], -1)[:, None, :]
+        radial = np.stack([cx, cz + apex], -1)[:, None, :]
+        np.testing.assert_allclose((d * radial).sum(-1), 0.0, atol=1e-15, err_msg=who)
+        np.testing.assert_allclose(np.hypot(d[..., 0], d[..., 1]), np.abs(o)[None, :] * np.ones((E, 1)), rtol=1e-12,
+                                   atol=1e-16, err_msg=who)
+        chord = np.stack([cx[2:] - cx[:-2], cz[2:] - cz[:-2]], -1)
+        chord /= np.hypot(chord[:, 0], chord[:, 1])[:, None]
+        np.testing.assert_allclose(d[1:-1], o[None, :, None] * chord[:, None, :], atol=1e-15, err_msg=who)
+
+
 # ----------------------------------------------------------------- plan (reference known answers)
 def test_plan_p4_focused_frequency_count():
     """tests/test_simus.py:355-386: ~812 +- 20 selected frequencies."""
@@ -119,6 +176,15 @@ def test_obliquity_per_baffle_and_horizon():
     # z = 0 beside the element: |theta| = pi/2 -> 1e-16 floor for every baffle
     for g in (soft, rigid, imp):
         assert abs(g["amp"][1, 0, 0]) < 1e-14
+
+
+def test_green_step_unit_modulus_without_attenuation():
+    """tests/test_pfield_helpers.py known answer: |step| = 1 with no attenuation; with attenuation it is below 1
+    and falls with distance."""
+    r = np.array([1e-4, 3e-3, 0.02, 0.11])
+    np.testing.assert_allclose(np.abs(O.green_step(r, 4321.0, 1540.0, 0.0)), 1.0, atol=1e-15)
+    a = np.abs(O.green_step(r, 4321.0, 1540.0, 0.5))
+    assert np.all(a < 1.0) and np.all(np.diff(a) < 0)
 
 
 def test_subelement_offsets():
@@ -267,6 +333,30 @@ def test_pfield_on_axis_focus_peak():
     pr = O.pfield_oracle(x, z, d, p, fs.MediumParams(), plan)
     assert 0.012 < z[np.argmax(pr)] < 0.028
     assert np.all(pr > 0)
+
+
+def test_pfield_plan_step_hand_computed():
+    """pfield.py:276-282: df = step / (max clipped sub-element distance / c + max delay), then the grid of
+    2 ceil(fc / df) + 1 points on [0, 2 fc]. One on-axis point, four elements, two sub-elements: the largest
+    distance is to the outermost sub-element, computed here by hand (float32 element centre + W/4)."""
+    p = fs.TransducerParams(freq_center=5e6, pitch=3.0e-4, width=2.7e-4, n_elements=4, bandwidth=0.6)
+    m = fs.MediumParams(speed_of_sound=1540.0)
+    x, z = np.array([0.0]), np.array([0.03])
+    d = np.array([0.0, 1e-7, 2e-7, 5e-8])
+    xe = float(np.float32(1.5) * np.float32(3.0e-4))
+    rmax = np.hypot(xe + 2.7e-4 / 4, 0.03)
+    df_max = 1.0 / (rmax / 1540.0 + 2e-7)
+    n = 2 * int(np.ceil(5e6 / df_max)) + 1
+    want = 2 * 5e6 / (n - 1)
+    o = O.pfield_plan(x, z, d, p, m, element_splitting=2)
+    assert o["df"] == pytest.approx(want, rel=1e-14)
+    assert o["corr"] == pytest.approx(want * 2.7e-4, rel=1e-14)
+    prod = fs.pfield_precompute(np.stack([x, z], -1), d, p, m, element_splitting=2)
+    assert prod.selected_freqs[1] - prod.selected_freqs[0] == pytest.approx(want, rel=1e-9)
+    assert prod.correction_factor == pytest.approx(want * 2.7e-4, rel=1e-14)
+    # the frequency_step option scales the largest admissible step
+    o2 = O.pfield_plan(x, z, d, p, m, element_splitting=2, frequency_step=0.5)
+    assert o2["df"] == pytest.approx(2 * 5e6 / (2 * int(np.ceil(5e6 / (0.5 * df_max)))), rel=1e-14)
 
 
 def _pp_power(plan):
