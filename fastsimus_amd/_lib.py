@@ -14,7 +14,8 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libfastsimus.so"
 
-FS_OK, FS_ERR_INVALID, FS_ERR_UNSUPPORTED, FS_ERR_WORKSPACE, FS_ERR_LAUNCH, FS_ERR_RESOURCE = range(6)
+FS_OK, FS_ERR_INVALID, FS_ERR_UNSUPPORTED, FS_ERR_WORKSPACE, FS_ERR_LAUNCH, FS_ERR_RESOURCE, FS_ERR_SCRATCH = range(7)
+FS_STATIC_SCRATCH_BYTES = 64 << 20
 FS_METHOD_FP32, FS_METHOD_MFMA_BF16X3 = 0, 1
 FS_ADJOINT_AUTO, FS_ADJOINT_MATRIX, FS_ADJOINT_VECTOR = 0, 1, 2
 
@@ -24,6 +25,7 @@ EXPORTS = (
     "fs_status_string",
     "fs_simus_workspace_bytes",
     "fs_simus_spectrum",
+    "fs_simus_spectrum_static",
     "fs_simus_batch_workspace_bytes",
     "fs_simus_spectrum_batch",
     "fs_tuning_set",
@@ -86,6 +88,8 @@ def load() -> C.CDLL:
     lib.fs_simus_workspace_bytes.argtypes = [D, C.c_int]
     lib.fs_simus_spectrum.restype = C.c_int
     lib.fs_simus_spectrum.argtypes = [D, C.c_int, P, P, C.c_size_t, P]
+    lib.fs_simus_spectrum_static.restype = C.c_int
+    lib.fs_simus_spectrum_static.argtypes = [D, P, P]
     I64 = C.POINTER(C.c_int64)
     lib.fs_simus_batch_workspace_bytes.restype = C.c_size_t
     lib.fs_simus_batch_workspace_bytes.argtypes = [D, C.c_int, C.c_int, I64]

@@ -3,6 +3,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "fs_device.h"
 
@@ -10,6 +11,7 @@ namespace fs {
 struct Fp32Plan;
 int fp32_spectrum(const Problem& p, cf* spect, char* ws, hipStream_t st);
 size_t fp32_workspace(const Problem& p);
+int fp32_spectrum_within(const Problem& p, cf* spect, char* ws, size_t ws_bytes, hipStream_t st);
 int mfma_spectrum(const Problem& p, cf* spect, char* ws, size_t ws_bytes, hipStream_t st);
 size_t mfma_workspace(const Problem& p);
 int mfma_spectrum_batch(const Problem& p, int P, const int64_t* off, cf* spect, char* ws, size_t ws_bytes,
@@ -32,6 +34,30 @@ int fill_zero(void* ptr, size_t n_words, hipStream_t st);
 }  // namespace fs
 
 using namespace fs;
+
+// ---------------------------------------------------------------- static scratch of fs_simus_spectrum_static
+// One array per GPU (each device holds its own copy at its own address). In host code `scratch` is
+// only a stand-in for the symbol: its device address is looked up once per device with
+// hipGetSymbolAddress and that pointer is what the kernels receive.
+static __device__ __align__(256) char scratch[FS_STATIC_SCRATCH_BYTES];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex scratch_mutex;
+static char* scratch_ptr[kMaxScratchDevices] = {};
+
+static int static_scratch(char** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return FS_ERR_LAUNCH;
+  if (dev < 0 || dev >= kMaxScratchDevices) return FS_ERR_RESOURCE;
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return FS_ERR_LAUNCH;
+    scratch_ptr[dev] = (char*)ptr;
+  }
+  *out = scratch_ptr[dev];
+  return FS_OK;
+}
 
 static int make_problem(const fs_simus_desc* d, Problem* p, bool need_rc, bool need_tx) {
   if (!d) return FS_ERR_INVALID;
@@ -91,6 +117,7 @@ const char* fs_status_string(int s) {
     case FS_ERR_WORKSPACE: return "workspace too small";
     case FS_ERR_LAUNCH: return "HIP launch or runtime error";
     case FS_ERR_RESOURCE: return "problem exceeds a kernel resource limit";
+    case FS_ERR_SCRATCH: return "problem exceeds the static scratch of fs_simus_spectrum_static (64 MiB)";
     default: return "unknown status";
   }
 }
@@ -122,6 +149,19 @@ int fs_simus_spectrum(const fs_simus_desc* d, int method, float* spect, void* wo
     return mfma_spectrum(p, (cf*)spect, (char*)workspace, workspace_bytes, st);
   }
   return FS_ERR_INVALID;
+}
+
+int fs_simus_spectrum_static(const fs_simus_desc* d, float* spect, void* stream) {
+  Problem p;
+  int rc = make_problem(d, &p, true, true);
+  if (rc != FS_OK) return rc;
+  if (!spect) return FS_ERR_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  if (p.S == 0) return fill_zero(spect, 2 * (size_t)p.NT * p.NF * p.E, st);
+  char* ws = nullptr;
+  rc = static_scratch(&ws);
+  if (rc != FS_OK) return rc;
+  return fp32_spectrum_within(p, (cf*)spect, ws, FS_STATIC_SCRATCH_BYTES, st);
 }
 
 static bool valid_offsets(const Problem& p, int P, const int64_t* off) {

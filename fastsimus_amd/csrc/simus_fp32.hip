@@ -261,10 +261,39 @@ __global__ __launch_bounds__(64 * RG) void k_reduce(Problem p, int nbuf, int npa
 // ----------------------------------------------------------------------------- host side
 struct Fp32Plan {
   int nfp, nfb, fbg, chunks, s_per_wave, nparts;
+  bool table_per_tx;  // the TX table holds one transmit, rebuilt before each (else all transmits at once)
   size_t off_A, off_pp, off_pr, off_Q, off_part, total;
 };
 
 static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// Enough waves per RX launch to fill 256 CUs about four times over at 4 waves per SIMD: a finer
+// split shortens the tail of the last round (50 010 scatterers: 9.40 ms with 8192 waves, 9.05 ms
+// with 16384, no further gain beyond); the partial sums grow with it (~265 MB there).
+static int default_chunks(const Problem& p, int fbg) {
+  const int etiles = (p.E + 63) / 64;
+  const long want_waves = 16384;
+  const long waves_per_chunk = (long)etiles * fbg * 4;
+  const int chunks = (int)std::max<long>(1, want_waves / std::max<long>(waves_per_chunk, 1));
+  const int max_chunks = std::max(1, (p.S + 4 * 16 - 1) / (4 * 16));  // at least 16 scatterers per wave
+  return std::min(chunks, max_chunks);
+}
+
+static void fp32_layout(const Problem& p, int fbg, int chunks, bool table_per_tx, Fp32Plan& P) {
+  P.fbg = fbg;
+  P.chunks = chunks;
+  P.table_per_tx = table_per_tx;
+  P.s_per_wave = (p.S + chunks * 4 - 1) / (chunks * 4);
+  P.nparts = chunks * 4;
+  const size_t per_block = (size_t)p.S * FB * sizeof(cf);
+  size_t off = 0;
+  P.off_A = off;    off = align_up(off + (size_t)(table_per_tx ? 1 : p.NT) * p.E * P.nfp * sizeof(cf));
+  P.off_pp = off;   off = align_up(off + (size_t)P.nfp * sizeof(cf));
+  P.off_pr = off;   off = align_up(off + (size_t)P.nfp * sizeof(float));
+  P.off_Q = off;    off = align_up(off + per_block * P.fbg);
+  P.off_part = off; off = align_up(off + (size_t)P.nparts * P.fbg * FB * p.E * sizeof(cf));
+  P.total = off;
+}
 
 Fp32Plan fp32_plan(const Problem& p) {
   Fp32Plan P{};
@@ -273,27 +302,36 @@ Fp32Plan fp32_plan(const Problem& p) {
   // Q buffer holds fbg frequency blocks for all scatterers: cap it near 2 GiB (of 288 GB of HBM), so
   // that large problems still get several blocks per group and few scatterer partitions to reduce.
   const size_t per_block = (size_t)p.S * FB * sizeof(cf);
-  P.fbg = (int)std::max<size_t>(1, std::min<size_t>((size_t)P.nfb, ((size_t)2 << 30) / std::max<size_t>(per_block, 1)));
-  const int etiles = (p.E + 63) / 64;
-  // enough waves per RX launch to fill 256 CUs about four times over at 4 waves per SIMD: a
-  // finer split shortens the tail of the last round (50 010 scatterers: 9.40 ms with 8192 waves,
-  // 9.05 ms with 16384, no further gain beyond); the partial sums grow with it (~265 MB there)
-  const long want_waves = 16384;
-  const long waves_per_chunk = (long)etiles * P.fbg * 4;
-  int chunks = (int)std::max<long>(1, want_waves / std::max<long>(waves_per_chunk, 1));
-  const int max_chunks = std::max(1, (p.S + 4 * 16 - 1) / (4 * 16));  // at least 16 scatterers per wave
-  chunks = std::min(chunks, max_chunks);
-  P.chunks = chunks;
-  P.s_per_wave = (p.S + chunks * 4 - 1) / (chunks * 4);
-  P.nparts = chunks * 4;
-  size_t off = 0;
-  P.off_A = off;    off = align_up(off + (size_t)p.NT * p.E * P.nfp * sizeof(cf));
-  P.off_pp = off;   off = align_up(off + (size_t)P.nfp * sizeof(cf));
-  P.off_pr = off;   off = align_up(off + (size_t)P.nfp * sizeof(float));
-  P.off_Q = off;    off = align_up(off + per_block * P.fbg);
-  P.off_part = off; off = align_up(off + (size_t)P.nparts * P.fbg * FB * p.E * sizeof(cf));
-  P.total = off;
+  const int fbg =
+      (int)std::max<size_t>(1, std::min<size_t>((size_t)P.nfb, ((size_t)2 << 30) / std::max<size_t>(per_block, 1)));
+  fp32_layout(p, fbg, default_chunks(p, fbg), false, P);
   return P;
+}
+
+// The same kernels within a scratch of at most `budget` bytes: the TX table holds one transmit,
+// then the largest frequency-block group (at most the default) whose default scatterer split
+// fits, else one block and as many scatterer chunks as fit. Returns false when even one block
+// with one chunk does not fit.
+bool fp32_plan_within(const Problem& p, size_t budget, Fp32Plan* out) {
+  Fp32Plan P{};
+  P.nfb = (p.NF + FB - 1) / FB;
+  P.nfp = P.nfb * FB;
+  const Fp32Plan def = fp32_plan(p);
+  for (int fbg = def.fbg; fbg >= 1; --fbg) {
+    fp32_layout(p, fbg, default_chunks(p, fbg), true, P);
+    if (P.total <= budget) {
+      *out = P;
+      return true;
+    }
+  }
+  fp32_layout(p, 1, 1, true, P);
+  if (P.total > budget) return false;
+  const size_t per_chunk = (size_t)4 * FB * p.E * sizeof(cf);
+  const int extra = (int)std::min<size_t>((budget - P.total) / per_chunk, (size_t)default_chunks(p, 1) - 1);
+  fp32_layout(p, 1, 1 + extra, true, P);
+  while (P.total > budget && P.chunks > 1) fp32_layout(p, 1, P.chunks - 1, true, P);  // alignment slack
+  *out = P;
+  return P.total <= budget;
 }
 
 // Two 32-frequency blocks per k_tx thread share its geometry but halve the grid and take about
@@ -341,40 +379,74 @@ static int dispatch_fd(const Problem& p, int tx, const Fp32Plan& P, int fb0, int
   return FS_OK;
 }
 
-int fp32_spectrum(const Problem& p, cf* spect, char* ws, hipStream_t st) {
-  const Fp32Plan P = fp32_plan(p);
+static void launch_tx_table_n(const Problem& p, int nfp, cf* A, hipStream_t st) {
+  const long n = (long)p.NT * p.E * nfp;
+  const int blocks = (int)std::min<long>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_tx_table, dim3(blocks), dim3(256), 0, st, p, nfp, A);
+}
+
+// All frequency-block groups of transmit `tx` of p (its TX table at A + tx E nfp), reduced into spect.
+static int fp32_transmit(const Problem& p, const Fp32Plan& P, int tx, const cf* A, const cf* pp, const float* pr, cf* Q,
+                         cf* part, cf* spect, hipStream_t st) {
+  for (int fb0 = 0; fb0 < P.nfb; fb0 += P.fbg) {
+    const int nb = std::min(P.fbg, P.nfb - fb0);
+    int rc;
+    switch (p.NS) {
+      case 1: rc = dispatch_fd<1>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
+      case 2: rc = dispatch_fd<2>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
+      case 3: rc = dispatch_fd<3>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
+      case 4: rc = dispatch_fd<4>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
+      default: return FS_ERR_UNSUPPORTED;
+    }
+    if (rc != FS_OK) return rc;
+    const int kb = fb0 * FB, ke = std::min((fb0 + nb) * FB, p.NF);
+    hipLaunchKernelGGL(k_reduce, dim3((p.E + 63) / 64, ke - kb), dim3(64, RG), 0, st, p, P.fbg * FB, P.nparts, kb,
+                       part, pr, tx, spect);
+  }
+  return FS_OK;
+}
+
+int fp32_spectrum_plan(const Problem& p, const Fp32Plan& P, cf* spect, char* ws, hipStream_t st) {
   cf* A = (cf*)(ws + P.off_A);
   cf* pp = (cf*)(ws + P.off_pp);
   float* pr = (float*)(ws + P.off_pr);
   cf* Q = (cf*)(ws + P.off_Q);
   cf* part = (cf*)(ws + P.off_part);
-  {
-    const long n = (long)p.NT * p.E * P.nfp;
-    const int blocks = (int)std::min<long>((n + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_tx_table, dim3(blocks), dim3(256), 0, st, p, P.nfp, A);
-    hipLaunchKernelGGL(k_spectra, dim3((P.nfp + 255) / 256), dim3(256), 0, st, p, P.nfp, pp, pr);
-  }
-  for (int tx = 0; tx < p.NT; ++tx) {
-    for (int fb0 = 0; fb0 < P.nfb; fb0 += P.fbg) {
-      const int nb = std::min(P.fbg, P.nfb - fb0);
-      int rc;
-      switch (p.NS) {
-        case 1: rc = dispatch_fd<1>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
-        case 2: rc = dispatch_fd<2>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
-        case 3: rc = dispatch_fd<3>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
-        case 4: rc = dispatch_fd<4>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
-        default: return FS_ERR_UNSUPPORTED;
-      }
+  hipLaunchKernelGGL(k_spectra, dim3((P.nfp + 255) / 256), dim3(256), 0, st, p, P.nfp, pp, pr);
+  if (!P.table_per_tx) {
+    launch_tx_table_n(p, P.nfp, A, st);
+    for (int tx = 0; tx < p.NT; ++tx) {
+      const int rc = fp32_transmit(p, P, tx, A, pp, pr, Q, part, spect, st);
       if (rc != FS_OK) return rc;
-      const int kb = fb0 * FB, ke = std::min((fb0 + nb) * FB, p.NF);
-      hipLaunchKernelGGL(k_reduce, dim3((p.E + 63) / 64, ke - kb), dim3(64, RG), 0, st, p, P.fbg * FB, P.nparts, kb,
-                         part, pr, tx, spect);
+    }
+  } else {
+    // transmit t as a one-transmit problem: its delays and apodization, its output rows
+    for (int tx = 0; tx < p.NT; ++tx) {
+      Problem pt = p;
+      pt.NT = 1;
+      pt.delay = p.delay + (size_t)tx * p.E;
+      pt.apod = p.apod + (size_t)tx * p.E;
+      launch_tx_table_n(pt, P.nfp, A, st);
+      const int rc = fp32_transmit(pt, P, 0, A, pp, pr, Q, part, spect + (size_t)tx * p.NF * p.E, st);
+      if (rc != FS_OK) return rc;
     }
   }
   return hipGetLastError() == hipSuccess ? FS_OK : FS_ERR_LAUNCH;
 }
 
+int fp32_spectrum(const Problem& p, cf* spect, char* ws, hipStream_t st) {
+  return fp32_spectrum_plan(p, fp32_plan(p), spect, ws, st);
+}
+
 size_t fp32_workspace(const Problem& p) { return fp32_plan(p).total; }
+
+// The plain method inside a fixed scratch of ws_bytes (no workspace query, no allocation):
+// FS_ERR_SCRATCH when the problem does not fit even with one block and one scatterer chunk.
+int fp32_spectrum_within(const Problem& p, cf* spect, char* ws, size_t ws_bytes, hipStream_t st) {
+  Fp32Plan P;
+  if (!fp32_plan_within(p, ws_bytes, &P)) return FS_ERR_SCRATCH;
+  return fp32_spectrum_plan(p, P, spect, ws, st);
+}
 
 // ----------------------------------------------------------------------------- pfield
 struct PfieldLayout {

@@ -45,7 +45,8 @@ enum fs_status {
   FS_ERR_UNSUPPORTED = 2,  /* option not implemented by this method -> NotImplementedError */
   FS_ERR_WORKSPACE = 3,    /* workspace smaller than required       -> RuntimeError */
   FS_ERR_LAUNCH = 4,       /* HIP launch / runtime error            -> RuntimeError */
-  FS_ERR_RESOURCE = 5      /* size beyond a kernel resource limit   -> RuntimeError */
+  FS_ERR_RESOURCE = 5,     /* size beyond a kernel resource limit   -> RuntimeError */
+  FS_ERR_SCRATCH = 6       /* problem larger than the static scratch of fs_simus_spectrum_static -> RuntimeError */
 };
 
 enum fs_method {
@@ -106,6 +107,20 @@ size_t fs_simus_workspace_bytes(const fs_simus_desc* d, int method);
  * Replaces simus_cuda(...) * plan.correction_factor. */
 int fs_simus_spectrum(const fs_simus_desc* d, int method, float* spect, void* workspace, size_t workspace_bytes,
                       void* stream);
+
+/* Allocation-free variant of fs_simus_spectrum with FS_METHOD_FP32. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory or pass a workspace (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation). Its scratch is one static device
+ * array of FS_STATIC_SCRATCH_BYTES per GPU, held from the library's load until the process ends;
+ * the array's address on the current device is looked up on the first call there (make that first
+ * call outside graph capture). The same kernels as fs_simus_spectrum run within it (the TX table is
+ * rebuilt per transmit; fewer frequency blocks and scatterer chunks per launch where the default
+ * plan does not fit), so results equal fs_simus_spectrum's up to FP32 summation order. Problems
+ * that do not fit even at one 32-frequency block and one scatterer chunk (about 250 000 scatterers
+ * at 128 elements and 1 900 frequencies) return FS_ERR_SCRATCH before any work. Every call on one
+ * GPU shares that GPU's scratch: calls that overlap on one GPU (other streams or threads) race. */
+#define FS_STATIC_SCRATCH_BYTES ((size_t)64 << 20)
+int fs_simus_spectrum_static(const fs_simus_desc* d, float* spect, void* stream);
 
 /* Batch of phantoms sharing the probe, the transmits and the frequency plan of d: the scatterers of
  * all phantoms are concatenated in d->scat_x / scat_z / rc (d->n_scat in total) and phantom q is
