@@ -6,9 +6,13 @@ speckle scatterers (x ~ U[-2, 2] cm, z ~ U[0.5, 5] cm, rc ~ N(0, 1), float32,
 seed 2026); 75 plane waves over [-16, 16] deg sharing one frequency plan.
 
 One step = the whole compute path for all 75 transmits with inputs resident
-in HBM: the split-BF16 MFMA spectrum (multistatic matrix + transmit apply)
-and the spectrum -> RF transform with the -100 dB gate. The plan is built once,
-outside the timed region (as the reference's own benchmark does).
+in HBM: the split-BF16 MFMA spectrum (multistatic matrix + transmit apply),
+the placed full spectrum (``SimusResult.spectrum``) and the spectrum -> RF
+transform with the -100 dB gate. The plan is built once, outside the timed
+region (as the reference's own benchmark does). By default the tile shapes are
+autotuned first and the step is captured in a HIP graph whose replays are
+timed; the dominant kernel's duration comes from HIP events around its eager
+launches in a second timed pass of the same step.
 
 Printed JSON (one line): ``value`` = whole-job evaluations/s in
 G(transmit . scatterer . element . frequency)/s for the MFMA path; beside it
@@ -64,6 +68,13 @@ def rel_l2(a, b) -> float:
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+def batched_useful_flops(n_scat: int, n_elem: int, n_freq: int, n_sub: int, n_tx: int) -> float:
+    """Algorithmic flops of the batched form: per (scatterer, element, frequency) 8 n_sub for the Green's
+    function (recurrence multiply + sub-element sum) and 16 per transmit (one complex MAC each for the TX and
+    the RX contraction); the Green's function is shared by every transmit."""
+    return float(n_scat) * n_elem * n_freq * (8 * n_sub + 16 * n_tx)
+
+
 def msm_flops_per_launch(n_scat: int, n_elem: int, n_freq: int, fbk: int = 4) -> float:
     """MFMA FLOPs issued by one k_msm launch (upper-triangular 16x16 tiles, 6 MFMAs each per 16 scatterers
     and frequency, frequencies padded to a multiple of the block depth fbk)."""
@@ -89,8 +100,9 @@ def measured_traffic(n_scat: int, n_tx: int, n_freq: int, n_elem: int):
     return d["fetch_bytes"] + d["write_bytes"]
 
 
-def cpu_baseline(params, medium, x, z, rc, delays, plan_o, n_sample: int, seed: int):
-    """FP64 NumPy oracle (the reference PYTHON strategy restated) on a bounded sample, 1 thread."""
+def cpu_baseline(params, medium, x, z, rc, delays, plan_o, n_sample: int, seed: int, repeats: int = 3):
+    """FP64 NumPy oracle (the reference PYTHON strategy restated) on a bounded sample, 1 thread, best of
+    ``repeats`` runs (plan excluded)."""
     from oracle import simus_oracle as O
 
     from threadpoolctl import threadpool_limits
@@ -99,15 +111,20 @@ def cpu_baseline(params, medium, x, z, rc, delays, plan_o, n_sample: int, seed: 
     idx = np.sort(rng.choice(x.size, n_sample, replace=False))
     xs, zs, rs = (a[idx].astype(np.float64) for a in (x, z, rc))
     t_mid = delays.shape[0] // 2
+    times = []
     with threadpool_limits(limits=1):
-        t0 = time.perf_counter()
-        rf, _, _ = O.simus_oracle(xs, zs, rs, delays[t_mid], params, medium, plan_o)
-        dt = time.perf_counter() - t0
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            rf, _, _ = O.simus_oracle(xs, zs, rs, delays[t_mid], params, medium, plan_o)
+            times.append(time.perf_counter() - t0)
+    dt = min(times)
     evals = n_sample * params.n_elements * plan_o["freqs"].size
     return {"value": evals / dt / 1e9, "unit": UNIT, "cores": 1, "kind": "port",
             "sample": f"{n_sample} of {x.size} scatterers, 1 of {delays.shape[0]} transmits (tx {t_mid}), "
-                      f"all {plan_o['freqs'].size} frequencies, FP64 NumPy, {dt:.1f} s, host {platform.processor() or platform.machine()} "
-                      f"({os.cpu_count()} logical CPUs visible)"}, idx, t_mid, rf
+                      f"all {plan_o['freqs'].size} frequencies, FP64 NumPy, best of {repeats}: {dt:.2f} s "
+                      f"(runs {', '.join(f'{t:.2f}' for t in times)} s), host "
+                      f"{platform.processor() or platform.machine()} ({os.cpu_count()} logical CPUs visible); "
+                      "rate per (scatterer, element, frequency) of one transmit"}, idx, t_mid, rf
 
 
 def phantom_workload(seed: int = 2027, n_phantoms: int = 64, n_beams: int = 128):
@@ -176,7 +193,8 @@ def bench_phantoms(args) -> int:
     k_avg = k_ms / max(k_n, 1) / 1e3
     tiles = sum(-(-int(off[q + 1] - off[q]) // 16) for q in range(P))
     nfpad = -(-nf // 4) * 4
-    flops = tiles * nfpad * 10 * 6 * (16 * 16 * 32 * 2)  # 4 element tiles: 10 upper tile pairs
+    issued = tiles * nfpad * 10 * 6 * (16 * 16 * 32 * 2)  # 4 element tiles: 10 upper tile pairs
+    useful = batched_useful_flops(S, E, nf, plan.n_sub, T)
     value = T * S * E * nf / t_step / 1e9
 
     # plain FP32 kernels on one beam of every phantom (cost linear in beams)
@@ -214,9 +232,11 @@ def bench_phantoms(args) -> int:
                    "n_phantoms": P, "n_scat_total": S, "n_scat_min": int(np.diff(off).min()),
                    "n_scat_max": int(np.diff(off).max()), "n_elem": E, "n_tx": T, "n_freq": nf,
                    "n_sub": plan.n_sub, "method": "hip_mfma"},
-        "roofline": {"bound": "mfma", "achieved": flops / k_avg / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
-                     "frac": flops / k_avg / PEAK_BF16, "traffic": None, "kernel": "k_msm (persistent, all phantoms)",
-                     "kernel_ms": k_avg * 1e3, "launches": k_n},
+        "roofline": {"bound": "mfma", "achieved": useful / k_avg / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
+                     "frac": useful / k_avg / PEAK_BF16, "traffic": None, "kernel": "k_msm (persistent, all phantoms)",
+                     "kernel_ms": k_avg * 1e3, "launches": k_n, "flops_per_launch": useful,
+                     "flops_definition": "S E n_f (8 n_sub + 16 n_tx) useful flops / k_msm HIP-event duration",
+                     "mfma_issued_flops_per_launch": issued, "mfma_issue_frac": issued / k_avg / PEAK_BF16},
         "plain_fp32": {"value": plain_value, "unit": UNIT, "transmits_timed": [T // 2]},
         "speedup_vs_plain": value / plain_value,
         "tuning": tuning,
@@ -228,6 +248,24 @@ def bench_phantoms(args) -> int:
     return 0
 
 
+def adjoint_workload(seed: int = 2028):
+    """Differentiable form: 128-element 7.6 MHz linear array, reflectivity on a 256 x 256 grid over
+    x in [-2, 2] cm, z in [0.5, 4.5] cm jittered by U[-0.05, 0.05] mm, rc ~ N(0, 1), 15 plane waves in
+    [-7, 7] deg, 0.5 dB/cm/MHz; the target phantom is rc + N(0, 0.1) (its RF taken at c = 1560 m/s)."""
+    import fastsimus_amd as fs
+
+    params = fs.TransducerParams(freq_center=7.6e6, pitch=3.0e-4, width=2.7e-4, n_elements=128, bandwidth=0.77)
+    rng = np.random.default_rng(seed)
+    gx, gz = np.meshgrid(np.linspace(-0.02, 0.02, 256), np.linspace(0.005, 0.045, 256), indexing="ij")
+    x = (gx.ravel() + rng.uniform(-5e-5, 5e-5, gx.size)).astype(np.float32)
+    z = (gz.ravel() + rng.uniform(-5e-5, 5e-5, gz.size)).astype(np.float32)
+    rc = rng.standard_normal(x.size).astype(np.float32)
+    rc_t = (rc + 0.1 * rng.standard_normal(x.size)).astype(np.float32)
+    pos, _, _ = fs.element_positions(params.n_elements, params.pitch, params.radius)
+    D = np.stack([fs.plane_wave(pos, float(a), speed_of_sound=1540.0) for a in np.deg2rad(np.linspace(-7, 7, 15))])
+    return params, x, z, rc, rc_t, D, 0.5
+
+
 def bench_adjoint(args) -> int:
     """Differentiable form: forward RF + gradient w.r.t. rc, x, z and c of an L2 loss, 256 x 256 jittered grid,
     15 plane waves in [-7, 7] deg; target from rc + N(0, 0.1) at c = 1560 m/s."""
@@ -237,16 +275,7 @@ def bench_adjoint(args) -> int:
     from fastsimus_amd.autograd import simus_rf
 
     dev = torch.device("cuda", 0)
-    params = fs.TransducerParams(freq_center=7.6e6, pitch=3.0e-4, width=2.7e-4, n_elements=128, bandwidth=0.77)
-    rng = np.random.default_rng(2028)
-    gx, gz = np.meshgrid(np.linspace(-0.02, 0.02, 256), np.linspace(0.005, 0.045, 256), indexing="ij")
-    x = (gx.ravel() + rng.uniform(-5e-5, 5e-5, gx.size)).astype(np.float32)
-    z = (gz.ravel() + rng.uniform(-5e-5, 5e-5, gz.size)).astype(np.float32)
-    rc = rng.standard_normal(x.size).astype(np.float32)
-    rc_t = (rc + 0.1 * rng.standard_normal(x.size)).astype(np.float32)
-    pos, _, _ = fs.element_positions(params.n_elements, params.pitch, params.radius)
-    D = np.stack([fs.plane_wave(pos, float(a), speed_of_sound=1540.0) for a in np.deg2rad(np.linspace(-7, 7, 15))])
-    att = 0.5
+    params, x, z, rc, rc_t, D, att = adjoint_workload()
     plan = fs.simus_precompute(np.stack([x, z], -1), rc, D, params, fs.MediumParams(attenuation=att))
     xt, zt = torch.tensor(x, device=dev), torch.tensor(z, device=dev)
     with torch.no_grad():
@@ -517,9 +546,10 @@ def main() -> int:
                          "pfield (RMS pressure field on a grid) or single (the single-transmit 2 000- and "
                          "50 010-scatterer cases)")
     ap.add_argument("--n-phantoms", type=int, default=64)
-    ap.add_argument("--tune", action="store_true",
-                    help="run the tile-shape autotuner before timing (default: library defaults, so that a kernel "
-                         "trace of the default command holds only the timed shape)")
+    ap.add_argument("--tune", action="store_true", help="phantoms64: run the tile-shape autotuner before timing")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="headline: time the library's default tile shapes instead of the autotuner's choice")
+    ap.add_argument("--no-graph", action="store_true", help="headline: time eager launches instead of graph replays")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (calibrates the FETCH_SIZE / WRITE_SIZE "
                          "counters of a profiler against a known byte count) and exit")
@@ -568,34 +598,62 @@ def main() -> int:
     lib = _lib.load()
     from fastsimus_amd import tune
 
-    tuning = tune.autotune(prob, reps=1) if args.tune else None
+    tuning = None if args.no_tune else tune.autotune(prob, reps=1)
+    # one step: spectrum (multistatic matrix + apply), placed full spectrum and gated RF of every transmit
+    bufs = {}
 
     def step():
         sp = K.run_spectrum(prob, "mfma")
-        rf, _, _ = K.run_rf(sp, plan.n_freq_full, plan.freq_idx_start, plan.n_fft, want_full=False)
-        return sp, rf
+        rf, full, _ = K.run_rf(sp, plan.n_freq_full, plan.freq_idx_start, plan.n_fft, want_full=True)
+        bufs["rf"], bufs["full"] = rf, full
+        return rf
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    graph = None
+    if not args.no_graph:
+        # capture on a side stream after the eager warm-up (kernel attributes set, code objects loaded);
+        # kernel timing is off while capturing
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            rf_m = step()
+        graph.replay()
+        torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        if graph is not None:
+            graph.replay()
+        else:
+            rf_m = step()
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    # the same step eager, its k_msm launches bracketed by HIP events on their stream
     lib.fs_set_kernel_timing(1)
     _lib.kernel_time_ms()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        sp_m, rf_m = step()
+        rf_e = step()
     torch.cuda.synchronize()
-    t_step = (time.perf_counter() - t0) / args.steps
+    t_eager = (time.perf_counter() - t0) / args.steps
     k_ms, k_n = _lib.kernel_time_ms()
     lib.fs_set_kernel_timing(0)
+    graph_vs_eager = float((rf_m - rf_e).abs().max()) if graph is not None else 0.0
     evals = T * S * E * nf
     value = evals / t_step / 1e9
     k_avg = k_ms / max(k_n, 1) / 1e3
     fbk = (tuning["chosen"].get("msm_fbk") or 4) if tuning else 4
-    flops = msm_flops_per_launch(S, E, nf, fbk)
-    roof = {"bound": "mfma", "achieved": flops / k_avg / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
-            "frac": flops / k_avg / PEAK_BF16, "traffic": measured_traffic(S, T, nf, E),
-            "kernel": "k_msm (split-BF16 v_mfma_f32_16x16x32_bf16)", "kernel_ms": k_avg * 1e3, "launches": k_n}
+    issued = msm_flops_per_launch(S, E, nf, fbk)
+    useful = batched_useful_flops(S, E, nf, plan.n_sub, T)
+    roof = {"bound": "mfma", "achieved": useful / k_avg / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
+            "frac": useful / k_avg / PEAK_BF16, "traffic": measured_traffic(S, T, nf, E),
+            "kernel": "k_msm (split-BF16 v_mfma_f32_16x16x32_bf16)", "kernel_ms": k_avg * 1e3, "launches": k_n,
+            "flops_per_launch": useful,
+            "flops_definition": "S E n_f (8 n_sub + 16 n_tx) useful flops per launch (Green's function shared by "
+                                "all transmits) / k_msm HIP-event duration",
+            "mfma_issued_flops_per_launch": issued,
+            "mfma_issue_tflops": issued / k_avg / 1e12, "mfma_issue_frac": issued / k_avg / PEAK_BF16}
 
     # plain FP32 kernel on a transmit subset of the same inputs
     sub = np.linspace(0, T - 1, max(1, args.plain_tx)).round().astype(int)
@@ -651,7 +709,10 @@ def main() -> int:
         "config": {"workload": "batched plane-wave compounding: 128-element 7.6 MHz linear array, 75 plane waves "
                                "in [-16,16] deg, 200000 speckle scatterers, 0.5 dB/cm/MHz",
                    "n_scat": S, "n_elem": E, "n_tx": T, "n_freq": nf, "n_sub": plan.n_sub, "n_fft": plan.n_fft,
-                   "method": "hip_mfma"},
+                   "method": "hip_mfma", "tiles": "autotuned" if tuning else "library defaults",
+                   "launch": "hipGraph replay" if graph is not None else "eager",
+                   "step": "spectrum + placed full spectrum + gated RF, all transmits"},
+        "eager_ms_per_step": t_eager * 1e3, "graph_vs_eager_rf_max_abs_diff": graph_vs_eager,
         "roofline": roof, "plain_fp32": plain, "speedup_vs_plain": value / plain_value, "rf_rel_l2": err,
         "tuning": tuning,
         "cpu_baseline": cpu,

@@ -11,8 +11,9 @@ correction factor, as for any fixed discretisation.
 
 The reference has no gradient kernel; its only differentiable path is the
 JAX scan strategy (``_simus_strategies.py:152-210``), whose algebra this
-differentiates. Full-frequency directivity is not supported here
-(``NotImplementedError``).
+differentiates, including its per-frequency directivity branch
+(``full_frequency_directivity``, ``_simus_strategies.py:177-186``; FP32 vector
+adjoint kernels).
 """
 
 from __future__ import annotations
@@ -26,7 +27,7 @@ from .plan import SimusPlan
 
 
 class _State:
-    def __init__(self, params, plan, delays, apod, attenuation, method, adjoint):
+    def __init__(self, params, plan, delays, apod, attenuation, method, adjoint, fulldir=False):
         self.params = params
         self.plan = plan
         self.delays = delays
@@ -34,6 +35,7 @@ class _State:
         self.attenuation = attenuation
         self.method = method
         self.adjoint = adjoint
+        self.fulldir = fulldir
 
 
 # fs_adjoint_method codes, passed with every fs_simus_adjoint call
@@ -52,7 +54,7 @@ def _problem(st: _State, x, z, rc, c: float):
     return K.Problem(pts, rc.detach(), st.delays, st.apod, st.params, medium, n_sub=st.plan.n_sub,
                      seg_length=st.plan.seg_length, freqs=st.plan.selected_freqs, pulse=st.plan.pulse_spectrum,
                      probe=st.plan.probe_spectrum, correction=st.plan.correction_factor,
-                     full_frequency_directivity=False, device=dev)
+                     full_frequency_directivity=st.fulldir, device=dev)
 
 
 class _SimusRF(torch.autograd.Function):
@@ -102,7 +104,7 @@ class _SimusRF(torch.autograd.Function):
 
 def simus_rf(x: torch.Tensor, z: torch.Tensor, rc: torch.Tensor, speed_of_sound, delays, params: TransducerParams,
              plan: SimusPlan, *, attenuation: float = 0.0, tx_apodization=None, method: str = "auto",
-             adjoint: str = "auto") -> torch.Tensor:
+             adjoint: str = "auto", full_frequency_directivity: bool = False) -> torch.Tensor:
     """Gated RF ``(n_tx, n_samples, n_elements)``, differentiable in x, z, rc and speed_of_sound.
 
     ``method`` selects the forward: ``"fp32"`` (per-transmit FP32 kernels), ``"mfma"`` (split-BF16
@@ -111,6 +113,9 @@ def simus_rf(x: torch.Tensor, z: torch.Tensor, rc: torch.Tensor, speed_of_sound,
     contractions as split-BF16 MFMA products, Green's functions generated once for up to 16
     transmits; at most 128 elements and 2 sub-elements), ``"vector"`` (FP32, Green's functions
     regenerated per transmit; any size) or ``"auto"`` (``"matrix"`` where it applies).
+    ``full_frequency_directivity`` evaluates the element directivity at every frequency instead of the
+    centre frequency (``_simus_strategies.py:177-186``); its gradient runs on the vector kernels
+    (``adjoint="matrix"`` raises ``NotImplementedError``).
     """
     if adjoint not in _ADJOINT:
         raise ValueError(f"unknown adjoint {adjoint!r}")
@@ -127,4 +132,5 @@ def simus_rf(x: torch.Tensor, z: torch.Tensor, rc: torch.Tensor, speed_of_sound,
     if method not in ("fp32", "mfma"):
         raise ValueError(f"unknown method {method!r}")
     return _SimusRF.apply(x, z, rc, speed_of_sound,
-                          _State(params, plan, delays, tx_apodization, attenuation, method, adjoint))
+                          _State(params, plan, delays, tx_apodization, attenuation, method, adjoint,
+                                 bool(full_frequency_directivity)))

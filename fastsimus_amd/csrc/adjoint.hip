@@ -35,12 +35,30 @@ constexpr int AFB = 32;
 struct Dcoef {
   float ax, az, ac;  // real parts
   cf bx, bz, bc;     // per-Hz parts
+  float fx, fz, fc;  // per-frequency directivity only: factors of sum_f f D'(y_f) E W (y_f = kappa_per_hz f sin(theta))
 };
+
+// sin(y) / y and its derivative, with the hardware sine / cosine in revolutions
+__device__ __forceinline__ void sinc_slope(float y, float& D, float& Dp) {
+  if (fabsf(y) < 1e-3f) {
+    D = 1.0f - y * y * (1.0f / 6.0f);
+    Dp = -y * (1.0f / 3.0f);
+  } else {
+    const float rev = y * (0.5f / kPi);
+    const float iy = __builtin_amdgcn_rcpf(y);
+    D = __builtin_amdgcn_sinf(rev) * iy;
+    Dp = (__builtin_amdgcn_cosf(rev) - D) * iy;
+  }
+}
 
 // amp, propagation and their derivatives for one (s, e, j) term, with the gfx950 hardware
 // transcendentals (1/r from the FP32 estimate plus one FP64 Newton step, v_sin / v_cos in
 // revolutions, reciprocals) and the propagation constants of GeoF (attenuation pre-multiplied
-// by log2 e).
+// by log2 e). With FULLDIR the directivity is a factor D(y_f) of every frequency, not part of
+// amp: amp and the a / b coefficients leave it out, and f_th = amp kappa_per_hz d(sin theta)/d th
+// (d(sin theta)/dc = 0, but y_f scales as 1/c: f_c = -amp kappa_per_hz sin(theta) / c) multiply
+// sum_f f D'(y_f) E W.
+template <bool FULLDIR = false>
 __device__ __forceinline__ void coeffs_fast(const Problem& p, double xs, double zs, int e, int j, float fc_hz,
                                             GeoF& geo, Dcoef& dc) {
   const int idx = e * p.NS + j;
@@ -79,16 +97,8 @@ __device__ __forceinline__ void coeffs_fast(const Problem& p, double xs, double 
     dobl = 0.0f;
   }
   const float yk = p.kappa * st;
-  float D, dDdy;
-  if (fabsf(yk) < 1e-3f) {
-    D = 1.0f - yk * yk / 6.0f;
-    dDdy = -yk / 3.0f;
-  } else {
-    const float rev = yk * (0.5f / kPi);
-    const float iy = __builtin_amdgcn_rcpf(yk);
-    D = __builtin_amdgcn_sinf(rev) * iy;
-    dDdy = (__builtin_amdgcn_cosf(rev) - D) * iy;
-  }
+  float D = 1.0f, dDdy = 0.0f;
+  if constexpr (!FULLDIR) sinc_slope(yk, D, dDdy);
   const float rcf = (float)rcl;
   const float rs = __builtin_amdgcn_rsqf(rcf) * p.inv_nsub;
   const float amp = obl * D * rs;
@@ -109,6 +119,14 @@ __device__ __forceinline__ void coeffs_fast(const Problem& p, double xs, double 
   const float dD_dc = dDdy * st * (-p.kappa * inv_c);
   dc.ac = obl * dD_dc * rs + amp * half * rhoc;
   dc.bc = {amp * kp.x * rhoc, amp * (kp.y * rhoc - 2.0f * kPi * rcf * inv_c * inv_c)};
+  if constexpr (FULLDIR) {
+    const float ak = amp * p.kappa_per_hz;
+    dc.fx = ak * ct * thx;
+    dc.fz = ak * ct * thz;
+    dc.fc = -ak * st * inv_c;
+  } else {
+    dc.fx = dc.fz = dc.fc = 0.0f;
+  }
   geo.t = rcl * p.inv_c;
   geo.amp = amp;
   geo.att = p.att_coef * rcf * kLog2e;
@@ -149,7 +167,7 @@ __global__ void k_adj_tables(Problem p, int nfp, const cf* __restrict__ gspect, 
 }
 
 // P_t (without rc) and B_t for one transmit; drc_part[fb][s] = sum_k Re(c_k P B)
-template <int NSUB>
+template <int NSUB, bool FULLDIR>
 __global__ __launch_bounds__(256) void k_adj_tx(Problem p, int tx, int nfp, const cf* __restrict__ A,
                                                 const cf* __restrict__ Gc, const cf* __restrict__ ppc,
                                                 cf* __restrict__ Pb, cf* __restrict__ Bb, float* __restrict__ drc_part) {
@@ -168,11 +186,13 @@ __global__ __launch_bounds__(256) void k_adj_tx(Problem p, int tx, int nfp, cons
   const cf* Gt = Gc + (size_t)tx * p.E * nfp + k0;
   for (int e = 0; e < p.E; ++e) {
     cf g[NSUB], q[NSUB];
+    float st[NSUB];
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
       const GeoF geo = geometry_fast(p, xs, zs, e, j);
       g[j] = green_fast(geo, fstart, geo.amp);
       q[j] = green_step_fast(geo, p.df);
+      st[j] = geo.sint;
     }
     const cf* Ae = At + (size_t)e * nfp;
     const cf* Ge = Gt + (size_t)e * nfp;
@@ -181,8 +201,14 @@ __global__ __launch_bounds__(256) void k_adj_tx(Problem p, int tx, int nfp, cons
       cf G = {0.f, 0.f};
 #pragma unroll
       for (int j = 0; j < NSUB; ++j) {
-        G.x += g[j].x;
-        G.y += g[j].y;
+        if constexpr (FULLDIR) {
+          const float d = sinc_fast(p.kappa_per_hz * (float)(fstart + (double)k * p.df) * st[j]);
+          G.x = fmaf(g[j].x, d, G.x);
+          G.y = fmaf(g[j].y, d, G.y);
+        } else {
+          G.x += g[j].x;
+          G.y += g[j].y;
+        }
         g[j] = cmul(g[j], q[j]);
       }
       cmac(P[k], G, Ae[k]);
@@ -207,7 +233,7 @@ __global__ __launch_bounds__(256) void k_adj_tx(Problem p, int tx, int nfp, cons
   drc_part[(size_t)fb * p.S + s] = drc;
 }
 
-template <int NSUB>
+template <int NSUB, bool FULLDIR>
 __global__ __launch_bounds__(256) void k_adj_pos(Problem p, int tx, int nfp, const cf* __restrict__ TA,
                                                  const cf* __restrict__ TG, const cf* __restrict__ Pb,
                                                  const cf* __restrict__ Bb, float* __restrict__ dxp,
@@ -235,21 +261,25 @@ __global__ __launch_bounds__(256) void k_adj_pos(Problem p, int tx, int nfp, con
   const cf* TGt = TG + (size_t)tx * p.E * nfp + k0;
   for (int e = 0; e < p.E; ++e) {
     cf E[NSUB], q[NSUB];
+    float st[NSUB];
     Dcoef dc[NSUB];
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
       GeoF geo;
-      coeffs_fast(p, xs, zs, e, j, fc_hz, geo, dc[j]);
+      coeffs_fast<FULLDIR>(p, xs, zs, e, j, fc_hz, geo, dc[j]);
       E[j] = green_fast(geo, fstart, 1.0f);  // E_j carries the propagation factor only
       q[j] = green_step_fast(geo, p.df);
+      st[j] = geo.sint;
     }
     const cf* TAe = TAt + (size_t)e * nfp;
     const cf* TGe = TGt + (size_t)e * nfp;
     // dg/dth = E (a + b f) is linear in f: accumulate S0 = sum_k E W and S1 = sum_k f E W over the
-    // block, then Re(dg W) summed over k = a Re(S0) + Re(b S1).
-    cf S0[NSUB], S1[NSUB];
+    // block, then Re(dg W) summed over k = a Re(S0) + Re(b S1). With the directivity at every
+    // frequency, dg/dth = E (D(y_f) (a + b f) + f_th f D'(y_f)): S0 and S1 carry the factor D(y_f)
+    // and S2 = sum_k f D'(y_f) E W adds f_th Re(S2).
+    cf S0[NSUB], S1[NSUB], S2[NSUB];
 #pragma unroll
-    for (int j = 0; j < NSUB; ++j) S0[j] = S1[j] = {0.f, 0.f};
+    for (int j = 0; j < NSUB; ++j) S0[j] = S1[j] = S2[j] = {0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < AFB; ++k) {
       const float f = (float)(fstart + (double)k * p.df);
@@ -258,7 +288,15 @@ __global__ __launch_bounds__(256) void k_adj_pos(Problem p, int tx, int nfp, con
       cmac(W, P[k], TGe[k]);
 #pragma unroll
       for (int j = 0; j < NSUB; ++j) {
-        const cf EW = cmul(E[j], W);
+        cf EW = cmul(E[j], W);
+        if constexpr (FULLDIR) {
+          float D, Dp;
+          sinc_slope(p.kappa_per_hz * f * st[j], D, Dp);
+          const float fdp = f * Dp;
+          S2[j].x = fmaf(fdp, EW.x, S2[j].x);
+          S2[j].y = fmaf(fdp, EW.y, S2[j].y);
+          EW = {EW.x * D, EW.y * D};
+        }
         S0[j].x += EW.x;
         S0[j].y += EW.y;
         S1[j].x = fmaf(f, EW.x, S1[j].x);
@@ -271,6 +309,11 @@ __global__ __launch_bounds__(256) void k_adj_pos(Problem p, int tx, int nfp, con
       gx += dc[j].ax * S0[j].x + (dc[j].bx.x * S1[j].x - dc[j].bx.y * S1[j].y);
       gz += dc[j].az * S0[j].x + (dc[j].bz.x * S1[j].x - dc[j].bz.y * S1[j].y);
       gc += dc[j].ac * S0[j].x + (dc[j].bc.x * S1[j].x - dc[j].bc.y * S1[j].y);
+      if constexpr (FULLDIR) {
+        gx += dc[j].fx * S2[j].x;
+        gz += dc[j].fz * S2[j].x;
+        gc += dc[j].fc * S2[j].x;
+      }
     }
   }
   if (!valid) return;
@@ -362,7 +405,7 @@ __global__ void k_ppc(Problem p, int nfp, cf* __restrict__ ppc) {
 // 16 w + 4 (l / 16) + c, c < 4: the A-operand layout of G in Y = G R and the
 // result layout of W^T = R L^T, so that G, E_j and W of one term sit in the
 // same lane. Y is summed over the waves through LDS.
-constexpr int AM_FC = 62;    // frequencies per work item = the FP64 reseed interval of the recurrence (3 k + 2)
+constexpr int AM_FC = 32;    // frequencies per work item = the FP64 reseed interval of the recurrence (3 k + 2)
 constexpr int AM_TG = 16;    // transmits per group
 constexpr int AM_MAXW = 8;   // element slices per workgroup (E <= 128)
 constexpr int AM_NSUB_MAX = 2;
@@ -823,7 +866,7 @@ static AdjLayout adj_layout(const Problem& p) {
 }
 
 
-template <int NSUB>
+template <int NSUB, bool FD>
 static void adj_launch(const Problem& p, const AdjLayout& L, char* ws, int t, float* d_rc, float* d_x, float* d_z,
                        hipStream_t st) {
   const dim3 grid((p.S + 255) / 256, L.nfb);
@@ -838,8 +881,8 @@ static void adj_launch(const Problem& p, const AdjLayout& L, char* ws, int t, fl
   float* dx = (float*)(ws + L.odx);
   float* dz = (float*)(ws + L.odz);
   float* dcp = (float*)(ws + L.odc) + (size_t)t * L.nfb * p.S;
-  hipLaunchKernelGGL(k_adj_tx<NSUB>, grid, dim3(256), 0, st, p, t, L.nfp, A, Gc, ppc, Pb, Bb, drc);
-  hipLaunchKernelGGL(k_adj_pos<NSUB>, grid, dim3(256), 0, st, p, t, L.nfp, TA, TG, Pb, Bb, dx, dz, dcp);
+  hipLaunchKernelGGL((k_adj_tx<NSUB, FD>), grid, dim3(256), 0, st, p, t, L.nfp, A, Gc, ppc, Pb, Bb, drc);
+  hipLaunchKernelGGL((k_adj_pos<NSUB, FD>), grid, dim3(256), 0, st, p, t, L.nfp, TA, TG, Pb, Bb, dx, dz, dcp);
   const dim3 g1((p.S + 255) / 256);
   hipLaunchKernelGGL(k_sum_parts, g1, dim3(256), 0, st, drc, L.nfb, p.S, t > 0 ? 1 : 0, d_rc);
   hipLaunchKernelGGL(k_sum_parts, g1, dim3(256), 0, st, dx, L.nfb, p.S, t > 0 ? 1 : 0, d_x);
@@ -858,11 +901,12 @@ static int adjoint_vector(const Problem& p, const cf* dspect, float* d_rc, float
     hipLaunchKernelGGL(k_ppc, dim3((L.nfp + 255) / 256), dim3(256), 0, st, p, L.nfp, (cf*)(ws + L.oppc));
   }
   for (int t = 0; t < p.NT; ++t) {
+    const bool fd = p.fulldir != 0;
     switch (p.NS) {
-      case 1: adj_launch<1>(p, L, ws, t, d_rc, d_x, d_z, st); break;
-      case 2: adj_launch<2>(p, L, ws, t, d_rc, d_x, d_z, st); break;
-      case 3: adj_launch<3>(p, L, ws, t, d_rc, d_x, d_z, st); break;
-      case 4: adj_launch<4>(p, L, ws, t, d_rc, d_x, d_z, st); break;
+      case 1: fd ? adj_launch<1, true>(p, L, ws, t, d_rc, d_x, d_z, st) : adj_launch<1, false>(p, L, ws, t, d_rc, d_x, d_z, st); break;
+      case 2: fd ? adj_launch<2, true>(p, L, ws, t, d_rc, d_x, d_z, st) : adj_launch<2, false>(p, L, ws, t, d_rc, d_x, d_z, st); break;
+      case 3: fd ? adj_launch<3, true>(p, L, ws, t, d_rc, d_x, d_z, st) : adj_launch<3, false>(p, L, ws, t, d_rc, d_x, d_z, st); break;
+      case 4: fd ? adj_launch<4, true>(p, L, ws, t, d_rc, d_x, d_z, st) : adj_launch<4, false>(p, L, ws, t, d_rc, d_x, d_z, st); break;
       default: return FS_ERR_UNSUPPORTED;
     }
   }
@@ -949,8 +993,8 @@ static int adjoint_matrix(const Problem& p, const cf* dspect, float* d_rc, float
 }
 
 // method (fs_adjoint_method, passed with every call): FS_ADJOINT_AUTO = matrix cores where they apply
-// (n_elem <= 128, n_sub <= 2), else vector; FS_ADJOINT_MATRIX = matrix cores only; FS_ADJOINT_VECTOR = vector only.
-static bool matrix_applies(const Problem& p) { return p.E <= 16 * AM_MAXW && p.NS <= AM_NSUB_MAX; }
+// (n_elem <= 128, n_sub <= 2, centre-frequency directivity), else vector; FS_ADJOINT_MATRIX = matrix cores only; FS_ADJOINT_VECTOR = vector only.
+static bool matrix_applies(const Problem& p) { return p.E <= 16 * AM_MAXW && p.NS <= AM_NSUB_MAX && !p.fulldir; }
 
 static bool use_matrix(const Problem& p, int method) { return method != FS_ADJOINT_VECTOR && matrix_applies(p); }
 

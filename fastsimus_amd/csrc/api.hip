@@ -290,7 +290,6 @@ int fs_simus_adjoint(const fs_simus_desc* d, int adjoint_method, const float* d_
   if (rc != FS_OK) return rc;
   if (!valid_adjoint_method(adjoint_method)) return FS_ERR_INVALID;
   if (!d_spect || !d_rc || !d_x || !d_z || !d_c) return FS_ERR_INVALID;
-  if (p.fulldir) return FS_ERR_UNSUPPORTED;
   if (!workspace || workspace_bytes < adjoint_workspace(p, adjoint_method)) return FS_ERR_WORKSPACE;
   return adjoint_fp32(p, adjoint_method, (const cf*)d_spect, d_rc, d_x, d_z, d_c, (char*)workspace,
                       (hipStream_t)stream);

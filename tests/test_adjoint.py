@@ -31,47 +31,57 @@ def _scene(n=10, seed=21):
     return x, z, rc, rc_t, D
 
 
-def _target(x, z, rc_t, D, plan):
+def _target(x, z, rc_t, D, plan, fulldir=False):
     with torch.no_grad():
         return simus_rf_torch(torch.as_tensor(x), torch.as_tensor(z), torch.as_tensor(rc_t), 1560.0, D, PARAMS, ATT,
-                              plan)
+                              plan, full_frequency_directivity=fulldir)
 
 
-def _oracle_grads(x, z, rc, D, plan, target):
+def _oracle_grads(x, z, rc, D, plan, target, fulldir=False):
     xt = torch.tensor(x, requires_grad=True)
     zt = torch.tensor(z, requires_grad=True)
     rt = torch.tensor(rc, requires_grad=True)
     ct = torch.tensor(1540.0, dtype=torch.float64, requires_grad=True)
-    rf = simus_rf_torch(xt, zt, rt, ct, D, PARAMS, ATT, plan)
+    rf = simus_rf_torch(xt, zt, rt, ct, D, PARAMS, ATT, plan, full_frequency_directivity=fulldir)
     loss = 0.5 * ((rf - target) ** 2).sum()
     loss.backward()
     return float(loss.detach()), rt.grad.numpy(), xt.grad.numpy(), zt.grad.numpy(), float(ct.grad)
 
 
-def test_torch_oracle_forward_matches_numpy_oracle():
+@pytest.mark.parametrize("fulldir", [False, True])
+def test_torch_oracle_forward_matches_numpy_oracle(fulldir):
     x, z, rc, _, D = _scene(4)
     m = fs.MediumParams(attenuation=ATT)
     plan = O.simus_plan(x, z, D, PARAMS, m)
-    rf_t = simus_rf_torch(torch.as_tensor(x), torch.as_tensor(z), torch.as_tensor(rc), 1540.0, D, PARAMS, ATT, plan)
+    rf_t = simus_rf_torch(torch.as_tensor(x), torch.as_tensor(z), torch.as_tensor(rc), 1540.0, D, PARAMS, ATT, plan,
+                          full_frequency_directivity=fulldir)
     for t in range(2):
-        rf_n, _, _ = O.simus_oracle(x, z, rc, D[t], PARAMS, m, plan)
+        rf_n, _, _ = O.simus_oracle(x, z, rc, D[t], PARAMS, m, plan, full_frequency_directivity=fulldir)
         assert rel_l2(rf_t[t].numpy(), rf_n) < 1e-9
+    if fulldir:  # the per-frequency sinc is a different forward, not a rescaling of the centre-frequency one
+        rf_c = simus_rf_torch(torch.as_tensor(x), torch.as_tensor(z), torch.as_tensor(rc), 1540.0, D, PARAMS, ATT,
+                              plan)
+        assert rel_l2(rf_c.numpy(), rf_t.numpy()) > 1e-3
 
 
-def test_torch_oracle_gradient_matches_finite_differences():
+@pytest.mark.parametrize("fulldir", [False, True])
+def test_torch_oracle_gradient_matches_finite_differences(fulldir):
     x, z, rc, rc_t, D = _scene(3)
     plan = O.simus_plan(x, z, D, PARAMS, fs.MediumParams(attenuation=ATT))
-    target = _target(x, z, rc_t, D, plan)
-    L, g_rc, g_x, g_z, g_c = _oracle_grads(x, z, rc, D, plan, target)
+    target = _target(x, z, rc_t, D, plan, fulldir)
+    L, g_rc, g_x, g_z, g_c = _oracle_grads(x, z, rc, D, plan, target, fulldir)
 
     def loss(xv, zv, rv, cv):
         with torch.no_grad():
-            rf = simus_rf_torch(torch.as_tensor(xv), torch.as_tensor(zv), torch.as_tensor(rv), cv, D, PARAMS, ATT, plan)
+            rf = simus_rf_torch(torch.as_tensor(xv), torch.as_tensor(zv), torch.as_tensor(rv), cv, D, PARAMS, ATT, plan,
+                                full_frequency_directivity=fulldir)
             return float(0.5 * ((rf - target) ** 2).sum())
 
+    # FP64 steps of 1e-8 relative: the gate 0.5 (1 + tanh((|rf| / max - 1e-5) / 1e-6)) turns over within 1e-6
+    # of the peak amplitude, so a step of 1e-6 relative can cross its curvature near threshold samples
     for i in range(3):
         for arr, g in ((x, g_x), (z, g_z), (rc, g_rc)):
-            h = 1e-6 * max(abs(arr[i]), 1e-3)
+            h = 1e-8 * max(abs(arr[i]), 1e-3)
             a, b = arr.copy(), arr.copy()
             a[i] += h
             b[i] -= h
@@ -81,10 +91,10 @@ def test_torch_oracle_gradient_matches_finite_differences():
             args_p = [a if j == k else v for j, v in enumerate(args_p)]
             args_m = [b if j == k else v for j, v in enumerate(args_m)]
             fd = (loss(*args_p, 1540.0) - loss(*args_m, 1540.0)) / (2 * h)
-            assert fd == pytest.approx(g[i], rel=2e-3, abs=1e-6 * abs(L) / max(abs(arr[i]), 1e-3))
-    h = 1e-6 * 1540.0
+            assert fd == pytest.approx(g[i], rel=1e-4, abs=1e-6 * abs(L) / max(abs(arr[i]), 1e-3))
+    h = 1e-8 * 1540.0
     fd = (loss(x, z, rc, 1540.0 + h) - loss(x, z, rc, 1540.0 - h)) / (2 * h)
-    assert fd == pytest.approx(g_c, rel=2e-3)
+    assert fd == pytest.approx(g_c, rel=1e-4)
 
 
 @pytest.mark.gpu
@@ -117,7 +127,44 @@ def test_hip_adjoint_matches_fp64_gradient(method, adjoint, cuda):
 
 
 @pytest.mark.gpu
-def test_hip_adjoint_matches_finite_differences(cuda):
+@pytest.mark.parametrize("method", ["fp32", "mfma"])
+def test_hip_adjoint_full_frequency_directivity_matches_fp64_gradient(method, cuda):
+    """Directivity at every frequency (the reference scan's branch, _simus_strategies.py:177-186): the vector
+    adjoint against the FP64 oracle gradient; the matrix-core adjoint does not take it."""
+    from fastsimus_amd.autograd import simus_rf
+
+    x, z, rc, rc_t, D = _scene(10, seed=23)
+    sc = np.stack([x, z], -1)
+    plan = fs.simus_precompute(sc, rc, D, PARAMS, fs.MediumParams(attenuation=ATT))
+    plan_o = O.simus_plan(x, z, D, PARAMS, fs.MediumParams(attenuation=ATT))
+    target = _target(x, z, rc_t, D, plan_o, fulldir=True)
+    L, g_rc, g_x, g_z, g_c = _oracle_grads(x, z, rc, D, plan_o, target, fulldir=True)
+    xt = torch.tensor(x, dtype=torch.float32, device=cuda, requires_grad=True)
+    zt = torch.tensor(z, dtype=torch.float32, device=cuda, requires_grad=True)
+    rt = torch.tensor(rc, dtype=torch.float32, device=cuda, requires_grad=True)
+    ct = torch.tensor(1540.0, dtype=torch.float64, requires_grad=True)
+    print("fulldir", method, flush=True)
+    rf = simus_rf(xt, zt, rt, ct, D, PARAMS, plan, attenuation=ATT, method=method, full_frequency_directivity=True)
+    tgt = target.to(torch.float32).to(cuda)
+    loss = 0.5 * ((rf - tgt) ** 2).sum()
+    loss.backward()
+    print("loss", float(loss.detach()), L, flush=True)
+    assert float(loss.detach()) == pytest.approx(L, rel=1e-3)
+    errs = [rel_l2(rt.grad.cpu().numpy(), g_rc), rel_l2(xt.grad.cpu().numpy(), g_x),
+            rel_l2(zt.grad.cpu().numpy(), g_z)]
+    print("rel-L2 rc, x, z", errs, "c", float(ct.grad), g_c, flush=True)
+    assert max(errs) < 1e-3
+    assert float(ct.grad) == pytest.approx(g_c, rel=1e-3)
+    rt2 = torch.tensor(rc, dtype=torch.float32, device=cuda, requires_grad=True)
+    rf2 = simus_rf(xt.detach(), zt.detach(), rt2, 1540.0, D, PARAMS, plan, attenuation=ATT, method=method,
+                   adjoint="matrix", full_frequency_directivity=True)
+    with pytest.raises(NotImplementedError):
+        rf2.sum().backward()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fulldir", [False, True])
+def test_hip_adjoint_matches_finite_differences(fulldir, cuda):
     """Directional check on a linear functional L = <w, rf> of the FP32 GPU forward.
 
     The steps (1e-4 relative for positions and c) keep the loss change far above
@@ -136,7 +183,8 @@ def test_hip_adjoint_matches_finite_differences(cuda):
     w = torch.as_tensor(np.random.default_rng(5).standard_normal((2, nkeep, 128)), dtype=torch.float64, device=cuda)
     with torch.no_grad():
         rf0 = simus_rf(torch.tensor(x, dtype=torch.float32, device=cuda), torch.tensor(z, dtype=torch.float32, device=cuda),
-                       torch.tensor(rc, dtype=torch.float32, device=cuda), 1540.0, D, PARAMS, plan, attenuation=ATT)
+                       torch.tensor(rc, dtype=torch.float32, device=cuda), 1540.0, D, PARAMS, plan, attenuation=ATT,
+                       full_frequency_directivity=fulldir)
         w = w * (rf0.abs() > 1e-3 * rf0.abs().amax(dim=(1, 2), keepdim=True)).double()
 
     def run(xv, zv, rv, c, grad=False):
@@ -144,7 +192,7 @@ def test_hip_adjoint_matches_finite_differences(cuda):
         zt = torch.tensor(zv, dtype=torch.float32, device=cuda, requires_grad=grad)
         rt = torch.tensor(rv, dtype=torch.float32, device=cuda, requires_grad=grad)
         ct = torch.tensor(c, dtype=torch.float64, requires_grad=grad)
-        rf = simus_rf(xt, zt, rt, ct, D, PARAMS, plan, attenuation=ATT)
+        rf = simus_rf(xt, zt, rt, ct, D, PARAMS, plan, attenuation=ATT, full_frequency_directivity=fulldir)
         loss = (rf.double() * w).sum()
         if grad:
             loss.backward()
