@@ -45,6 +45,7 @@ _ADJOINT = {"auto": _lib.FS_ADJOINT_AUTO, "matrix": _lib.FS_ADJOINT_MATRIX, "vec
 # From this many transmits on, the multistatic-matrix (MFMA) forward, whose cost does not grow
 # with the transmit count, is faster than the per-transmit FP32 kernels.
 MFMA_MIN_TX = 3
+MFMA_MAX_ELEMENTS = 1024  # element limit of the batched (multistatic-matrix) method
 
 
 def _problem(st: _State, x, z, rc, c: float):
@@ -108,7 +109,7 @@ def simus_rf(x: torch.Tensor, z: torch.Tensor, rc: torch.Tensor, speed_of_sound,
     """Gated RF ``(n_tx, n_samples, n_elements)``, differentiable in x, z, rc and speed_of_sound.
 
     ``method`` selects the forward: ``"fp32"`` (per-transmit FP32 kernels), ``"mfma"`` (split-BF16
-    multistatic matrix, at most 128 elements) or ``"auto"`` (``"mfma"`` from ``MFMA_MIN_TX``
+    multistatic matrix, at most 1024 elements) or ``"auto"`` (``"mfma"`` from ``MFMA_MIN_TX``
     transmits on where it applies). ``adjoint`` selects the backward kernels: ``"matrix"`` (element
     contractions as split-BF16 MFMA products, Green's functions generated once for up to 16
     transmits; at most 128 elements and 2 sub-elements), ``"vector"`` (FP32, Green's functions
@@ -128,7 +129,7 @@ def simus_rf(x: torch.Tensor, z: torch.Tensor, rc: torch.Tensor, speed_of_sound,
     if method == "auto":
         n_tx = 1 if (delays.dim() if isinstance(delays, torch.Tensor) else len(getattr(delays, "shape", ()))) < 2 \
             else int(delays.shape[0])
-        method = "mfma" if (n_tx >= MFMA_MIN_TX and params.n_elements <= 128) else "fp32"
+        method = "mfma" if (n_tx >= MFMA_MIN_TX and params.n_elements <= MFMA_MAX_ELEMENTS) else "fp32"
     if method not in ("fp32", "mfma"):
         raise ValueError(f"unknown method {method!r}")
     return _SimusRF.apply(x, z, rc, speed_of_sound,

@@ -35,6 +35,14 @@ from .simus import _method
 MAX_PHANTOMS_PER_CALL = 192  # fs_simus_spectrum_batch limit; larger batches are split
 
 
+def max_phantoms_per_call(n_elements: int, method: str) -> int:
+    """Phantoms per fs_simus_spectrum_batch call: 192 work units per launch, and above 128 elements the
+    batched method takes one unit per pair of 64-element groups for every phantom."""
+    groups = -(-n_elements // 64)
+    pairs = 1 if groups <= 2 else groups * (groups - 1) // 2
+    return MAX_PHANTOMS_PER_CALL if method != "mfma" else max(1, MAX_PHANTOMS_PER_CALL // pairs)
+
+
 class BatchResult(NamedTuple):
     """``rf`` (n_phantoms, n_tx, n_samples, n_elements) float32; ``spectrum`` (n_phantoms, n_tx,
     n_freq_full, n_elements) complex64 or None; ``offsets`` (n_phantoms + 1,) into the concatenation."""
@@ -101,8 +109,9 @@ def simus_batch_compute(phantoms: Sequence, delays, plan: SimusPlan, params: Tra
     pts, rc, offsets = _concat(phantoms)
     dev = K.hip_device(pts, rc, delays, tx_apodization)
     rfs, fulls = [], []
-    for q0 in range(0, len(offsets) - 1, MAX_PHANTOMS_PER_CALL):
-        q1 = min(q0 + MAX_PHANTOMS_PER_CALL, len(offsets) - 1)
+    per_call = max_phantoms_per_call(params.n_elements, method)
+    for q0 in range(0, len(offsets) - 1, per_call):
+        q1 = min(q0 + per_call, len(offsets) - 1)
         b, e = int(offsets[q0]), int(offsets[q1])
         prob = _problem(pts[b:e], rc[b:e], delays, tx_apodization, plan, params, medium, full_frequency_directivity,
                         dev)
@@ -140,7 +149,8 @@ class SimusBatchGraph:
     reflectivities (float32, same total length, same phantom boundaries) into
     the graph's static inputs, replays the graph on the current stream and
     returns the static RF tensor ``(n_phantoms, n_tx, n_samples, n_elements)``
-    (overwritten by the next replay). At most 192 phantoms. Kernel timing
+    (overwritten by the next replay). At most 192 phantoms (fewer above 128 elements with the batched
+    method, ``max_phantoms_per_call``). Kernel timing
     (``fs_set_kernel_timing``) must be off while capturing.
     """
 
@@ -149,8 +159,9 @@ class SimusBatchGraph:
                  full_frequency_directivity: bool = False, strategy="hip_mfma"):
         method = _method(strategy)
         pts, rc, offsets = _concat(phantoms)
-        if len(offsets) - 1 > MAX_PHANTOMS_PER_CALL:
-            raise NotImplementedError(f"a captured batch holds at most {MAX_PHANTOMS_PER_CALL} phantoms")
+        per_call = max_phantoms_per_call(params.n_elements, method)
+        if len(offsets) - 1 > per_call:
+            raise NotImplementedError(f"a captured batch holds at most {per_call} phantoms for this array")
         dev = K.hip_device(pts, rc, delays, tx_apodization)
         self.offsets = offsets
         self.plan = plan

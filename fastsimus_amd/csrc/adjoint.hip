@@ -166,8 +166,15 @@ __global__ void k_adj_tables(Problem p, int nfp, const cf* __restrict__ gspect, 
   }
 }
 
+// Sub-elements in groups of NSUB: with GROUPED the problem has p.NS of them and those past p.NS get zero
+// amplitude and coefficients; without it p.NS == NSUB.
+template <int NSUB, bool GROUPED>
+__device__ __forceinline__ int adj_groups(const Problem& p) {
+  return GROUPED ? (p.NS + NSUB - 1) / NSUB : 1;
+}
+
 // P_t (without rc) and B_t for one transmit; drc_part[fb][s] = sum_k Re(c_k P B)
-template <int NSUB, bool FULLDIR>
+template <int NSUB, bool FULLDIR, bool GROUPED = false>
 __global__ __launch_bounds__(256) void k_adj_tx(Problem p, int tx, int nfp, const cf* __restrict__ A,
                                                 const cf* __restrict__ Gc, const cf* __restrict__ ppc,
                                                 cf* __restrict__ Pb, cf* __restrict__ Bb, float* __restrict__ drc_part) {
@@ -184,13 +191,17 @@ __global__ __launch_bounds__(256) void k_adj_tx(Problem p, int tx, int nfp, cons
   const double fstart = p.f0 + (double)k0 * p.df;
   const cf* At = A + (size_t)tx * p.E * nfp + k0;
   const cf* Gt = Gc + (size_t)tx * p.E * nfp + k0;
-  for (int e = 0; e < p.E; ++e) {
+  const int ngroups = adj_groups<NSUB, GROUPED>(p);
+  for (int e = 0; e < p.E; ++e)
+  for (int jg = 0; jg < ngroups; ++jg) {
     cf g[NSUB], q[NSUB];
     float st[NSUB];
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
-      const GeoF geo = geometry_fast(p, xs, zs, e, j);
-      g[j] = green_fast(geo, fstart, geo.amp);
+      const int jj = jg * NSUB + j;
+      const bool ok = !GROUPED || jj < p.NS;
+      const GeoF geo = geometry_fast(p, xs, zs, e, ok ? jj : 0);
+      g[j] = green_fast(geo, fstart, ok ? geo.amp : 0.f);
       q[j] = green_step_fast(geo, p.df);
       st[j] = geo.sint;
     }
@@ -233,7 +244,7 @@ __global__ __launch_bounds__(256) void k_adj_tx(Problem p, int tx, int nfp, cons
   drc_part[(size_t)fb * p.S + s] = drc;
 }
 
-template <int NSUB, bool FULLDIR>
+template <int NSUB, bool FULLDIR, bool GROUPED = false>
 __global__ __launch_bounds__(256) void k_adj_pos(Problem p, int tx, int nfp, const cf* __restrict__ TA,
                                                  const cf* __restrict__ TG, const cf* __restrict__ Pb,
                                                  const cf* __restrict__ Bb, float* __restrict__ dxp,
@@ -259,15 +270,19 @@ __global__ __launch_bounds__(256) void k_adj_pos(Problem p, int tx, int nfp, con
   const float fc_hz = (float)(p.min_dist > 0 ? 0.5 / (p.min_dist * p.inv_c) : 1.0);
   const cf* TAt = TA + (size_t)tx * p.E * nfp + k0;
   const cf* TGt = TG + (size_t)tx * p.E * nfp + k0;
-  for (int e = 0; e < p.E; ++e) {
+  const int ngroups = adj_groups<NSUB, GROUPED>(p);
+  for (int e = 0; e < p.E; ++e)
+  for (int jg = 0; jg < ngroups; ++jg) {
     cf E[NSUB], q[NSUB];
     float st[NSUB];
     Dcoef dc[NSUB];
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
+      const int jj = jg * NSUB + j;
+      const bool ok = !GROUPED || jj < p.NS;
       GeoF geo;
-      coeffs_fast<FULLDIR>(p, xs, zs, e, j, fc_hz, geo, dc[j]);
-      E[j] = green_fast(geo, fstart, 1.0f);  // E_j carries the propagation factor only
+      coeffs_fast<FULLDIR>(p, xs, zs, e, ok ? jj : 0, fc_hz, geo, dc[j]);
+      E[j] = green_fast(geo, fstart, ok ? 1.0f : 0.0f);  // E_j carries the propagation factor only
       q[j] = green_step_fast(geo, p.df);
       st[j] = geo.sint;
     }
@@ -866,7 +881,7 @@ static AdjLayout adj_layout(const Problem& p) {
 }
 
 
-template <int NSUB, bool FD>
+template <int NSUB, bool FD, bool GR = false>
 static void adj_launch(const Problem& p, const AdjLayout& L, char* ws, int t, float* d_rc, float* d_x, float* d_z,
                        hipStream_t st) {
   const dim3 grid((p.S + 255) / 256, L.nfb);
@@ -881,8 +896,8 @@ static void adj_launch(const Problem& p, const AdjLayout& L, char* ws, int t, fl
   float* dx = (float*)(ws + L.odx);
   float* dz = (float*)(ws + L.odz);
   float* dcp = (float*)(ws + L.odc) + (size_t)t * L.nfb * p.S;
-  hipLaunchKernelGGL((k_adj_tx<NSUB, FD>), grid, dim3(256), 0, st, p, t, L.nfp, A, Gc, ppc, Pb, Bb, drc);
-  hipLaunchKernelGGL((k_adj_pos<NSUB, FD>), grid, dim3(256), 0, st, p, t, L.nfp, TA, TG, Pb, Bb, dx, dz, dcp);
+  hipLaunchKernelGGL((k_adj_tx<NSUB, FD, GR>), grid, dim3(256), 0, st, p, t, L.nfp, A, Gc, ppc, Pb, Bb, drc);
+  hipLaunchKernelGGL((k_adj_pos<NSUB, FD, GR>), grid, dim3(256), 0, st, p, t, L.nfp, TA, TG, Pb, Bb, dx, dz, dcp);
   const dim3 g1((p.S + 255) / 256);
   hipLaunchKernelGGL(k_sum_parts, g1, dim3(256), 0, st, drc, L.nfb, p.S, t > 0 ? 1 : 0, d_rc);
   hipLaunchKernelGGL(k_sum_parts, g1, dim3(256), 0, st, dx, L.nfb, p.S, t > 0 ? 1 : 0, d_x);
@@ -907,7 +922,7 @@ static int adjoint_vector(const Problem& p, const cf* dspect, float* d_rc, float
       case 2: fd ? adj_launch<2, true>(p, L, ws, t, d_rc, d_x, d_z, st) : adj_launch<2, false>(p, L, ws, t, d_rc, d_x, d_z, st); break;
       case 3: fd ? adj_launch<3, true>(p, L, ws, t, d_rc, d_x, d_z, st) : adj_launch<3, false>(p, L, ws, t, d_rc, d_x, d_z, st); break;
       case 4: fd ? adj_launch<4, true>(p, L, ws, t, d_rc, d_x, d_z, st) : adj_launch<4, false>(p, L, ws, t, d_rc, d_x, d_z, st); break;
-      default: return FS_ERR_UNSUPPORTED;
+      default: fd ? adj_launch<4, true, true>(p, L, ws, t, d_rc, d_x, d_z, st) : adj_launch<4, false, true>(p, L, ws, t, d_rc, d_x, d_z, st); break;
     }
   }
   hipLaunchKernelGGL(k_sum_dc_segments, dim3(DC_SEGMENTS), dim3(256), 0, st, (const float*)(ws + L.odc),

@@ -35,6 +35,10 @@ int fill_zero(void* ptr, size_t n_words, hipStream_t st);
 
 using namespace fs;
 
+// Sub-elements per element: up to 4 in registers, more in groups of 4 (plain and vector-adjoint
+// kernels) or one at a time (matrix-core kernel); this bound only rejects absurd descriptors.
+constexpr int kMaxSubElements = 1024;
+
 // ---------------------------------------------------------------- static scratch of fs_simus_spectrum_static
 // One array per GPU (each device holds its own copy at its own address). In host code `scratch` is
 // only a stand-in for the symbol: its device address is looked up once per device with
@@ -62,7 +66,7 @@ static int static_scratch(char** out) {
 static int make_problem(const fs_simus_desc* d, Problem* p, bool need_rc, bool need_tx) {
   if (!d) return FS_ERR_INVALID;
   if (d->n_scat < 0 || d->n_elem <= 0 || d->n_sub <= 0 || d->n_freq <= 0 || d->n_tx <= 0) return FS_ERR_INVALID;
-  if (d->n_sub > 4) return FS_ERR_UNSUPPORTED;
+  if (d->n_sub > kMaxSubElements) return FS_ERR_UNSUPPORTED;
   if (d->n_scat > 0 && (!d->scat_x || !d->scat_z || (need_rc && !d->rc))) return FS_ERR_INVALID;
   if (!d->sub_x || !d->sub_z || !d->elem_cos || !d->elem_sin) return FS_ERR_INVALID;
   if (need_tx && (!d->tx_delay || !d->tx_apod)) return FS_ERR_INVALID;

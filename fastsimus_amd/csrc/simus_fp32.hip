@@ -63,12 +63,27 @@ __global__ void k_spectra(Problem p, int nfp, cf* __restrict__ pp, float* __rest
   }
 }
 
+// Sub-elements [j0, j0 + NSUB) of a group: with GROUPED the problem has p.NS sub-elements, taken
+// NSUB at a time, and those past p.NS get a zero amplitude (their terms are exactly 0); without it
+// p.NS == NSUB and there is one group.
+template <int NSUB, bool GROUPED>
+__device__ __forceinline__ int n_groups(const Problem& p) {
+  return GROUPED ? (p.NS + NSUB - 1) / NSUB : 1;
+}
+template <bool GROUPED>
+__device__ __forceinline__ GeoF group_geometry(const Problem& p, double xs, double zs, int e, int jj) {
+  const bool ok = !GROUPED || jj < p.NS;
+  GeoF g = geometry_fast(p, xs, zs, e, ok ? jj : 0);
+  if (!ok) g.amp = 0.f;
+  return g;
+}
+
 // A thread takes BPT consecutive 32-frequency blocks of the launch (lb = BPT blockIdx.y + b,
 // b < BPT, lb < nb), so that the per-(scatterer, element, sub-element) geometry and frequency
 // step are evaluated once for all of them; every block is still seeded from its own FP64 phase.
 // MODE 0 (simus): Q[lb][s][k] = rc_s PP_k P_k(s).
 // MODE 1 (pfield): pw[fb][s] = sum_{k in block} |PP_k P_k(s)|^2 (reference _freq_step_body, _pfield_strategies.py:21-50).
-template <int NSUB, bool FULLDIR, int MODE, int BPT>
+template <int NSUB, bool FULLDIR, int MODE, int BPT, bool GROUPED = false>
 __global__ __launch_bounds__(256) void k_tx(Problem p, int tx, int nfp, int fb0, int nb, const cf* __restrict__ A,
                                             const cf* __restrict__ pp, cf* __restrict__ Q, float* __restrict__ pw) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -86,12 +101,14 @@ __global__ __launch_bounds__(256) void k_tx(Problem p, int tx, int nfp, int fb0,
     for (int k = 0; k < FB; ++k) P[b][k] = {0.f, 0.f};
 
   const cf* At = A + (size_t)tx * p.E * nfp + (size_t)(fb0 + lb0) * FB;
-  for (int e = 0; e < p.E; ++e) {
+  const int ngroups = n_groups<NSUB, GROUPED>(p);
+  for (int e = 0; e < p.E; ++e)
+  for (int jg = 0; jg < ngroups; ++jg) {
     GeoF geo[NSUB];
     cf q[NSUB];
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
-      geo[j] = geometry_fast(p, xs, zs, e, j);
+      geo[j] = group_geometry<GROUPED>(p, xs, zs, e, jg * NSUB + j);
       q[j] = green_step_fast(geo[j], p.df);
     }
 #pragma unroll
@@ -156,7 +173,7 @@ __global__ void k_pfield_reduce(const float* __restrict__ pw, int nfb, int S, fl
 }
 
 // part[(chunk * 4 + wave)][k - k_begin][e], k_begin = first frequency of the block group
-template <int NSUB, bool FULLDIR>
+template <int NSUB, bool FULLDIR, bool GROUPED = false>
 __global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_per_wave, const cf* __restrict__ Q,
                                             cf* __restrict__ part) {
   const int lane = threadIdx.x & 63;
@@ -179,8 +196,10 @@ __global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_
   const int se = min(sb + s_per_wave, p.S);
   const cf* Qb = Q + (size_t)blockIdx.y * p.S * FB;
   SubGeo sg[NSUB];
+  if constexpr (!GROUPED) {
 #pragma unroll
-  for (int j = 0; j < NSUB; ++j) sg[j] = sub_geo(p, ee, j);
+    for (int j = 0; j < NSUB; ++j) sg[j] = sub_geo(p, ee, j);
+  }
   for (int s = sb; s < se; ++s) {
     const cf* Qs = Qb + (size_t)s * FB;
     if (p.rc[s] == 0.f) continue;  // wave-uniform: the scatterer contributes nothing
@@ -191,11 +210,19 @@ __global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_
     cf qv[FB];
 #pragma unroll
     for (int k = 0; k < FB; ++k) qv[k] = Qs[k];
+    const int ngroups = n_groups<NSUB, GROUPED>(p);
+    for (int jg = 0; jg < ngroups; ++jg) {
     cf g[NSUB], q[NSUB];
     GeoF geo[NSUB];
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
-      geo[j] = geometry_fast(p, xs, zs, sg[j]);
+      if constexpr (GROUPED) {
+        const int jj = jg * NSUB + j;
+        geo[j] = geometry_fast(p, xs, zs, sub_geo(p, ee, jj < p.NS ? jj : 0));
+        if (jj >= p.NS) geo[j].amp = 0.f;
+      } else {
+        geo[j] = geometry_fast(p, xs, zs, sg[j]);
+      }
       g[j] = green_fast(geo[j], fstart, geo[j].amp);
       q[j] = green_step_fast(geo[j], p.df);
     }
@@ -216,6 +243,7 @@ __global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_
       }
       cmac(X[k], G, qv[k]);
     }
+    }  // sub-element groups
   }
   if (!valid) return;
   cf* out = part + ((size_t)(chunk * 4 + wave) * nbuf + blockIdx.y * FB) * p.E + e;
@@ -341,12 +369,12 @@ bool fp32_plan_within(const Problem& p, size_t budget, Fp32Plan* out) {
 constexpr long kTwoBlockMinGroups = 2048;
 static bool two_blocks_per_thread(int sx, int nb) { return (long)sx * ((nb + 1) / 2) >= kTwoBlockMinGroups; }
 
-template <int NSUB, bool FD>
+template <int NSUB, bool FD, bool GR>
 static void launch_tx(const Problem& p, int tx, const Fp32Plan& P, int fb0, int nb, const cf* A, const cf* pp, cf* Q,
                       hipStream_t st) {
   const int sx = (p.S + 255) / 256;
   const int slot = timing_start(st);
-  if constexpr (!FD) {
+  if constexpr (!FD && !GR) {
     if (two_blocks_per_thread(sx, nb)) {
       hipLaunchKernelGGL((k_tx<NSUB, false, 0, 2>), dim3(sx, (nb + 1) / 2), dim3(256), 0, st, p, tx, P.nfp, fb0, nb, A,
                          pp, Q, (float*)nullptr);
@@ -354,27 +382,28 @@ static void launch_tx(const Problem& p, int tx, const Fp32Plan& P, int fb0, int 
       return;
     }
   }
-  hipLaunchKernelGGL((k_tx<NSUB, FD, 0, 1>), dim3(sx, nb), dim3(256), 0, st, p, tx, P.nfp, fb0, nb, A, pp, Q,
+  hipLaunchKernelGGL((k_tx<NSUB, FD, 0, 1, GR>), dim3(sx, nb), dim3(256), 0, st, p, tx, P.nfp, fb0, nb, A, pp, Q,
                      (float*)nullptr);
   timing_stop(slot, st);
 }
-template <int NSUB, bool FD>
+template <int NSUB, bool FD, bool GR>
 static void launch_rx(const Problem& p, const Fp32Plan& P, int fb0, int nb, const cf* Q, cf* part, hipStream_t st) {
   dim3 grid((p.E + 63) / 64, nb, P.chunks);
   const int slot = timing_start(st);
-  hipLaunchKernelGGL((k_rx<NSUB, FD>), grid, dim3(256), 0, st, p, P.fbg * FB, fb0, P.s_per_wave, Q, part);
+  hipLaunchKernelGGL((k_rx<NSUB, FD, GR>), grid, dim3(256), 0, st, p, P.fbg * FB, fb0, P.s_per_wave, Q, part);
   timing_stop(slot, st);
 }
 
-template <int NSUB>
+// NSUB sub-elements exactly, or (GR) any count in groups of NSUB
+template <int NSUB, bool GR = false>
 static int dispatch_fd(const Problem& p, int tx, const Fp32Plan& P, int fb0, int nb, const cf* A, const cf* pp, cf* Q,
                        cf* part, hipStream_t st) {
   if (p.fulldir) {
-    launch_tx<NSUB, true>(p, tx, P, fb0, nb, A, pp, Q, st);
-    launch_rx<NSUB, true>(p, P, fb0, nb, Q, part, st);
+    launch_tx<NSUB, true, GR>(p, tx, P, fb0, nb, A, pp, Q, st);
+    launch_rx<NSUB, true, GR>(p, P, fb0, nb, Q, part, st);
   } else {
-    launch_tx<NSUB, false>(p, tx, P, fb0, nb, A, pp, Q, st);
-    launch_rx<NSUB, false>(p, P, fb0, nb, Q, part, st);
+    launch_tx<NSUB, false, GR>(p, tx, P, fb0, nb, A, pp, Q, st);
+    launch_rx<NSUB, false, GR>(p, P, fb0, nb, Q, part, st);
   }
   return FS_OK;
 }
@@ -396,7 +425,7 @@ static int fp32_transmit(const Problem& p, const Fp32Plan& P, int tx, const cf* 
       case 2: rc = dispatch_fd<2>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
       case 3: rc = dispatch_fd<3>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
       case 4: rc = dispatch_fd<4>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
-      default: return FS_ERR_UNSUPPORTED;
+      default: rc = dispatch_fd<4, true>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
     }
     if (rc != FS_OK) return rc;
     const int kb = fb0 * FB, ke = std::min((fb0 + nb) * FB, p.NF);
@@ -469,12 +498,15 @@ static PfieldLayout pfield_layout(const Problem& p) {
 
 size_t pfield_workspace(const Problem& p) { return pfield_layout(p).total; }
 
-template <int NSUB>
+template <int NSUB, bool GR = false>
 static void launch_pfield(const Problem& p, const PfieldLayout& L, const cf* A, const cf* pp, float* pw,
                           hipStream_t st) {
   const int sx = (p.S + 255) / 256;
   if (p.fulldir)
-    hipLaunchKernelGGL((k_tx<NSUB, true, 1, 1>), dim3(sx, L.nfb), dim3(256), 0, st, p, 0, L.nfp, 0, L.nfb, A, pp,
+    hipLaunchKernelGGL((k_tx<NSUB, true, 1, 1, GR>), dim3(sx, L.nfb), dim3(256), 0, st, p, 0, L.nfp, 0, L.nfb, A, pp,
+                       (cf*)nullptr, pw);
+  else if (GR)
+    hipLaunchKernelGGL((k_tx<NSUB, false, 1, 1, GR>), dim3(sx, L.nfb), dim3(256), 0, st, p, 0, L.nfp, 0, L.nfb, A, pp,
                        (cf*)nullptr, pw);
   else if (two_blocks_per_thread(sx, L.nfb))
     hipLaunchKernelGGL((k_tx<NSUB, false, 1, 2>), dim3(sx, (L.nfb + 1) / 2), dim3(256), 0, st, p, 0, L.nfp, 0, L.nfb,
@@ -502,7 +534,7 @@ int pfield_fp32(const Problem& p, float* out, char* ws, hipStream_t st) {
     case 2: launch_pfield<2>(q, L, A, pp, pw, st); break;
     case 3: launch_pfield<3>(q, L, A, pp, pw, st); break;
     case 4: launch_pfield<4>(q, L, A, pp, pw, st); break;
-    default: return FS_ERR_UNSUPPORTED;
+    default: launch_pfield<4, true>(q, L, A, pp, pw, st); break;
   }
   hipLaunchKernelGGL(k_pfield_reduce, dim3((q.S + 255) / 256), dim3(256), 0, st, pw, L.nfb, q.S, q.corr, out);
   return hipGetLastError() == hipSuccess ? FS_OK : FS_ERR_LAUNCH;

@@ -189,19 +189,28 @@ __device__ __forceinline__ float sinc_poly_pi(float a) {
   return fmaf(-x, r, 1.0f);
 }
 
-// Per-thread element data, constant over the whole launch.
+// Per-thread element data, constant over the whole launch. NSUB = 0 stands for a sub-element count
+// known only at run time (p.NS, more than 4): then the centroids are read where they are used.
 template <int NSUB>
 struct ElemData {
-  double sx[NSUB], sz[NSUB];
+  double sx[NSUB > 0 ? NSUB : 1], sz[NSUB > 0 ? NSUB : 1];
   float ce, se;
-  int ee;  // element index clamped into range
+  int ee;     // element index clamped into range
+  int ephys;  // the thread's element (may be a padding index >= E, below the table's padded width)
   bool valid;
 };
 
+// Elements in groups of 64. The workgroup's element v (0 .. 16 ET - 1) is element v of group ga for
+// v < 64 and element v - 64 of group gb above: a work unit of a pair of groups (ga < gb) yields the
+// blocks (ga, ga), (ga, gb) and (gb, gb) of the multistatic matrix. With at most 128 elements the
+// pair is (0, 1) and the map is the identity.
+__device__ __forceinline__ int phys_element(int v, int ga, int gb) { return v < 64 ? 64 * ga + v : 64 * gb + (v - 64); }
+
 template <int NSUB, int ET>
-__device__ __forceinline__ ElemData<NSUB> load_element(const Problem& p) {
+__device__ __forceinline__ ElemData<NSUB> load_element(const Problem& p, int ga, int gb) {
   ElemData<NSUB> d;
-  const int e = threadIdx.x % (ET * 16);
+  const int e = phys_element(threadIdx.x % (ET * 16), ga, gb);
+  d.ephys = e;
   d.valid = e < p.E;
   const int ee = d.valid ? e : 0;
   d.ee = ee;
@@ -316,6 +325,7 @@ template <int NSUB, int FBK, int MODE, int ET>
 __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<NSUB>& ed, const BlockScalars& bs,
                                               const float4* __restrict__ scat, int s0, int s_end, u32x4* frag) {
   using L = FragLds<ET, FBK>;
+  const int ns = NSUB > 0 ? NSUB : p.NS;
   const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / (ET * 16)));  // wave-uniform
   const int e = threadIdx.x % (ET * 16);
   float re[FBK][4], im[FBK][4];
@@ -329,11 +339,11 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<N
     const double quarter = ws < 0.f ? 0.25 : 0.0;
     const double xd = (double)sd.x, zd = (double)sd.y;
 #pragma unroll
-    for (int j = 0; j < NSUB; ++j) {
+    for (int j = 0; j < ns; ++j) {
       // sub-element centroids: held in registers where they fit, else re-read (cached)
-      constexpr bool held = NSUB <= 2 && MODE == MSM_FAST;
-      const double sxj = held ? ed.sx[j] : p.sub_x[ed.ee * NSUB + j];
-      const double szj = held ? ed.sz[j] : p.sub_z[ed.ee * NSUB + j];
+      constexpr bool held = NSUB > 0 && NSUB <= 2 && MODE == MSM_FAST;
+      const double sxj = held ? ed.sx[j] : p.sub_x[ed.ee * ns + j];
+      const double szj = held ? ed.sz[j] : p.sub_z[ed.ee * ns + j];
       const double dx = xd - sxj;
       const double dz = zd - szj;
       double r2 = fma(dx, dx, dz * dz);
@@ -414,15 +424,15 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* base, int
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, kRawBufferWord3);
 }
 
-// The NSUB entries of table row `row` = s * EP (wave-uniform) for element e,
+// The NSUB entries of table row `row` = s * ept (wave-uniform; ept elements per row) for element e,
 // by buffer loads: scalar row address plus the lane's 32-bit byte offset.
-template <int NSUB, int EP>
+template <int NSUB>
 __device__ __forceinline__ TableTerm<NSUB> load_term(const double* __restrict__ rt, const float* __restrict__ la,
-                                                     const float2* __restrict__ qt, size_t row, int e) {
+                                                     const float2* __restrict__ qt, size_t row, int e, int ept) {
   TableTerm<NSUB> t;
-  const __amdgpu_buffer_rsrc_t rr = row_rsrc(rt + row * NSUB, EP * NSUB * 8);
-  const __amdgpu_buffer_rsrc_t rl = row_rsrc(la + 2 * row * NSUB, EP * NSUB * 8);
-  const __amdgpu_buffer_rsrc_t rq = row_rsrc(qt + row * NSUB, EP * NSUB * 8);
+  const __amdgpu_buffer_rsrc_t rr = row_rsrc(rt + row * NSUB, ept * NSUB * 8);
+  const __amdgpu_buffer_rsrc_t rl = row_rsrc(la + 2 * row * NSUB, ept * NSUB * 8);
+  const __amdgpu_buffer_rsrc_t rq = row_rsrc(qt + row * NSUB, ept * NSUB * 8);
   if constexpr (NSUB == 2) {
     const u32x4 r = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rr, e * 16, 0, 0));
     const u32x4 l = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rl, e * 16, 0, 0));
@@ -457,20 +467,20 @@ struct TableTile {
   int wbits[4];  // bits of w = +-sqrt|rc| / n_sub: the sign of rc (wave-uniform)
 };
 
+// ept: elements per table row (the element count padded to the element groups), e: the thread's element
 template <int NSUB, int ET>
 __device__ __forceinline__ void fetch_table(const float4* __restrict__ scat, const double* __restrict__ rt,
                                             const float* __restrict__ la, const float2* __restrict__ qt, int s0,
-                                            int s_end, int s_dead, TableTile<NSUB>& tt) {
+                                            int s_end, int s_dead, int ept, int e, TableTile<NSUB>& tt) {
   constexpr int EP = ET * 16;
   const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / EP));  // wave-uniform
-  const int e = threadIdx.x % EP;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int s = s0 + 4 * g + i;
     const bool valid = s < s_end;
     // scatterers past the unit's end read the table's dead row (amplitude 2^-inf = 0)
     tt.wbits[i] = __float_as_int(scat[valid ? s : s_end - 1].z);
-    tt.t[i] = load_term<NSUB, EP>(rt, la, qt, (size_t)(valid ? s : s_dead) * EP, e);
+    tt.t[i] = load_term<NSUB>(rt, la, qt, (size_t)(valid ? s : s_dead) * ept, e, ept);
   }
 }
 
@@ -618,13 +628,13 @@ struct WaveTiles {
 template <int NSUB, int ET, int MODE, bool SHALLOW, int W>
 __device__ __forceinline__ void msm_wave(const Problem& p, const float4* __restrict__ scat,
                                          const double* __restrict__ rt, const float* __restrict__ la,
-                                         const float2* __restrict__ qt, int sb, int se, int fb, int k0,
-                                         bool mfma_first, char* lds, cf* kpart) {
+                                         const float2* __restrict__ qt, int ept, int sb, int se, int ga, int gb,
+                                         int fb, int k0, bool mfma_first, char* lds, cf* kpart) {
   using SH = MsmShape<ET, SHALLOW>;
   constexpr int FBK = SH::FBK;
   using L = FragLds<ET, FBK>;
   u32x4* const frag = (u32x4*)lds;  // buffer b at frag + b * per_buf
-  const ElemData<NSUB> ed = load_element<NSUB, ET>(p);
+  const ElemData<NSUB> ed = load_element<NSUB, ET>(p, ga, gb);
   const BlockScalars bs = block_scalars(p, p.f0 + (double)(fb * FBK) * p.df);
   WaveTiles<ET, FBK, SH::KPW, W> t;
   t.zero();
@@ -632,7 +642,7 @@ __device__ __forceinline__ void msm_wave(const Problem& p, const float4* __restr
   if constexpr (MODE == MSM_TABLE) {
     TableTile<NSUB> nxt;
     if (sb < se) {
-      fetch_table<NSUB, ET>(scat, rt, la, qt, sb, se, p.S, nxt);
+      fetch_table<NSUB, ET>(scat, rt, la, qt, sb, se, p.S, ept, ed.ephys, nxt);
       gen_fragments_table<NSUB, FBK, ET>(bs, nxt, frag);
     }
     __syncthreads();
@@ -642,7 +652,7 @@ __device__ __forceinline__ void msm_wave(const Problem& p, const float4* __restr
       if (mfma_first) t.step(frag + buf * L::per_buf, k0);
       if (more) {
         if (!mfma_first) __builtin_amdgcn_s_setprio(1);
-        fetch_table<NSUB, ET>(scat, rt, la, qt, s0 + MST, se, p.S, nxt);
+        fetch_table<NSUB, ET>(scat, rt, la, qt, s0 + MST, se, p.S, ept, ed.ephys, nxt);
         gen_fragments_table<NSUB, FBK, ET>(bs, nxt, frag + nb * L::per_buf);
         __builtin_amdgcn_s_setprio(0);
       }
@@ -673,13 +683,16 @@ __device__ __forceinline__ void msm_wave(const Problem& p, const float4* __restr
 }
 
 // ---------------------------------------------------------------- work units
-// A unit is a contiguous scatterer range of one phantom with its own partial
-// K slot; an item is (unit, frequency block). Units are built on the host,
-// largest first, and passed by value (so that a captured graph replays them).
+// A unit is a contiguous scatterer range of one phantom and one pair of element groups
+// (ga | gb << 16) with its own partial K slot; an item is (unit, frequency block). Units are built
+// on the host, largest first, and passed by value (so that a captured graph replays them).
 struct MsmUnit {
-  int s_begin, s_end, slot, pad;
+  int s_begin, s_end, slot, pair;
 };
 constexpr int MSM_MAX_UNITS = 192;
+// Element limit of the batched method: 16 groups of 64, i.e. 120 pairs of groups, each a work unit of a
+// scatterer chunk (at most MSM_MAX_UNITS units per launch).
+constexpr int MSM_MAX_ELEMENTS = 1024;
 struct MsmUnits {
   int n;    // units
   int nfb;  // frequency blocks
@@ -688,9 +701,38 @@ struct MsmUnits {
 };
 constexpr int MSM_MAX_PHANTOMS = 192;
 struct MsmPhantoms {
-  int n, pad;
-  int2 ph[MSM_MAX_PHANTOMS];  // (first K slot, slot count) of each phantom
+  int n;       // phantoms
+  int groups;  // element groups of 64 (units per scatterer chunk: one, or one per pair of groups above 2 groups)
+  int2 ph[MSM_MAX_PHANTOMS];  // (first K slot, scatterer chunks) of each phantom: pair q, chunk c at first + q chunks + c
 };
+
+// Pair index of element groups a < b among G groups (pairs ordered (0,1), (0,2), .., (1,2), ..)
+__host__ __device__ __forceinline__ int group_pair_index(int a, int b, int G) { return a * G - a * (a + 1) / 2 + (b - a - 1); }
+
+// Where K[r][col] of physical elements lives among the partial matrices of the pairs of element groups:
+// the pair and the (virtual) row and column in its 16 ET x 16 ET matrix. Off-diagonal blocks come from
+// the pair of their two groups; the diagonal block of group g from the pair (g, g + 1), the last one's
+// from (G - 2, G - 1). With at most two groups there is one pair and the map is the identity.
+__device__ __forceinline__ void k_location(int r, int col, int G, int& pair, int& vr, int& vc) {
+  if (G <= 2) {
+    pair = 0;
+    vr = r;
+    vc = col;
+    return;
+  }
+  const int gr = r >> 6, gc = col >> 6;
+  int a, b;
+  if (gr == gc) {
+    a = gr + 1 < G ? gr : G - 2;
+    b = a + 1;
+  } else {
+    a = min(gr, gc);
+    b = max(gr, gc);
+  }
+  vr = (gr == a ? 0 : 64) + (r & 63);
+  vc = (gc == a ? 0 : 64) + (col & 63);
+  pair = group_pair_index(a, b, G);
+}
 
 // Persistent: every workgroup takes items from a queue (atomic counter, reset
 // by the host before the launch) until none is left; items come largest unit
@@ -700,7 +742,7 @@ template <int NSUB, int ET, int MODE, bool SHALLOW, int W>
 __device__ __forceinline__ void msm_persistent(const Problem& p, const float4* __restrict__ scat,
                                                const double* __restrict__ rt, const float* __restrict__ la,
                                                const float2* __restrict__ qt, const MsmUnits& units,
-                                               int* __restrict__ queue, int nfpad,
+                                               int* __restrict__ queue, int nfpad, int ept,
                                                cf* __restrict__ kpart, int k0, bool mfma_first, char* lds,
                                                int* next_item) {
   const int n_items = units.n * units.nfb;
@@ -714,7 +756,8 @@ __device__ __forceinline__ void msm_persistent(const Problem& p, const float4* _
     const int fb = item - ui * units.nfb;
     const MsmUnit u = units.u[ui];
     cf* kp = kpart + (size_t)u.slot * nfpad * (ET * 16) * (ET * 16);
-    msm_wave<NSUB, ET, MODE, SHALLOW, W>(p, scat, rt, la, qt, u.s_begin, u.s_end, fb, k0, mfma_first, lds, kp);
+    msm_wave<NSUB, ET, MODE, SHALLOW, W>(p, scat, rt, la, qt, ept, u.s_begin, u.s_end, u.pair & 0xffff, u.pair >> 16,
+                                         fb, k0, mfma_first, lds, kp);
   }
 }
 
@@ -726,7 +769,7 @@ __global__ __launch_bounds__(64 * ET, ET == 8 ? 1 : 2) void k_msm(Problem p, con
                                                                  const double* __restrict__ rt,
                                                                  const float* __restrict__ la,
                                                                  const float2* __restrict__ qt, const MsmUnits units,
-                                                                 int* __restrict__ queue, int nfpad,
+                                                                 int* __restrict__ queue, int nfpad, int ept,
                                                                  cf* __restrict__ kpart) {
   using SH = MsmShape<ET, SHALLOW>;
   extern __shared__ __attribute__((aligned(16))) char lds_raw[];
@@ -736,16 +779,16 @@ __global__ __launch_bounds__(64 * ET, ET == 8 ? 1 : 2) void k_msm(Problem p, con
   const bool mfma_first = ((w ^ (w >> 2)) & 1) == 0;
   if constexpr (SH::WPP == 4) {
     switch (w % 4) {
-      case 0: msm_persistent<NSUB, ET, MODE, SHALLOW, 0>(p, scat, rt, la, qt, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
-      case 1: msm_persistent<NSUB, ET, MODE, SHALLOW, 1>(p, scat, rt, la, qt, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
-      case 2: msm_persistent<NSUB, ET, MODE, SHALLOW, 2>(p, scat, rt, la, qt, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
-      default: msm_persistent<NSUB, ET, MODE, SHALLOW, 3>(p, scat, rt, la, qt, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item); break;
+      case 0: msm_persistent<NSUB, ET, MODE, SHALLOW, 0>(p, scat, rt, la, qt, units, queue, nfpad, ept, kpart, k0, mfma_first, lds_raw, &next_item); break;
+      case 1: msm_persistent<NSUB, ET, MODE, SHALLOW, 1>(p, scat, rt, la, qt, units, queue, nfpad, ept, kpart, k0, mfma_first, lds_raw, &next_item); break;
+      case 2: msm_persistent<NSUB, ET, MODE, SHALLOW, 2>(p, scat, rt, la, qt, units, queue, nfpad, ept, kpart, k0, mfma_first, lds_raw, &next_item); break;
+      default: msm_persistent<NSUB, ET, MODE, SHALLOW, 3>(p, scat, rt, la, qt, units, queue, nfpad, ept, kpart, k0, mfma_first, lds_raw, &next_item); break;
     }
   } else {
     if (w % 2 == 0)
-      msm_persistent<NSUB, ET, MODE, SHALLOW, 0>(p, scat, rt, la, qt, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item);
+      msm_persistent<NSUB, ET, MODE, SHALLOW, 0>(p, scat, rt, la, qt, units, queue, nfpad, ept, kpart, k0, mfma_first, lds_raw, &next_item);
     else
-      msm_persistent<NSUB, ET, MODE, SHALLOW, 1>(p, scat, rt, la, qt, units, queue, nfpad, kpart, k0, mfma_first, lds_raw, &next_item);
+      msm_persistent<NSUB, ET, MODE, SHALLOW, 1>(p, scat, rt, la, qt, units, queue, nfpad, ept, kpart, k0, mfma_first, lds_raw, &next_item);
   }
 }
 
@@ -757,9 +800,10 @@ __global__ void k_set_int(int* __restrict__ ptr, int value) {
 // out[t][k][e] = corr probe_k PP_k sum_e' K_k[e][e'] A_t(k, e'),  K_k = sum_c Kpart[c][k] (upper tiles, symmetric)
 constexpr int AP_CH = 32;  // e' per LDS chunk
 
-// Grid (frequency, phantom); phantom ph sums its K slots and writes out[ph][t][k][e].
-// MAXT transmits per thread and pass (2 MAXT per pass over the two thread parities): 40, or fewer
-// when the problem has fewer transmits, so that the per-thread accumulation does no padded work.
+// Grid (frequency, phantom, block of 128 elements); phantom ph sums its K slots and writes
+// out[ph][t][k][e] for the block's elements. MAXT transmits per thread and pass (2 MAXT per pass over
+// the two thread parities): 40, or fewer when the problem has fewer transmits, so that the per-thread
+// accumulation does no padded work.
 template <int MAXT>
 __global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, const MsmPhantoms phantoms, int nfpad,
                                                    const cf* __restrict__ kpart, cf* __restrict__ out_all) {
@@ -767,9 +811,11 @@ __global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, const M
   __shared__ cf As[AP_CH][2 * MAXT + 1];
   const int k = blockIdx.x;
   const int slot0 = phantoms.ph[blockIdx.y].x, nchunks = phantoms.ph[blockIdx.y].y;
+  const int G = phantoms.groups;
+  const int rbase = blockIdx.z * 128;
   cf* __restrict__ out = out_all + (size_t)blockIdx.y * p.NT * p.NF * p.E;
   const int tid = threadIdx.x;
-  const int e = tid & 127;
+  const int e = rbase + (tid & 127);
   const int tg = tid >> 7;  // transmit parity
   const double f = p.f0 + (double)k * p.df;
   for (int t0 = 0; t0 < p.NT; t0 += 2 * MAXT) {
@@ -779,21 +825,24 @@ __global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, const M
     for (int c0 = 0; c0 < p.E; c0 += AP_CH) {
       __syncthreads();
       for (int q = tid; q < 128 * AP_CH; q += 256) {
-        const int r = q / AP_CH, cc = q % AP_CH;
-        const int col = c0 + cc;
+        const int rr = q / AP_CH, cc = q % AP_CH;
+        const int r = rbase + rr, col = c0 + cc;
         cf v = {0.f, 0.f};
         if (r < p.E && col < p.E) {
-          const int ra = ((r >> 4) <= (col >> 4)) ? r : col;
-          const int cb = ((r >> 4) <= (col >> 4)) ? col : r;
+          int pair, vr, vc;
+          k_location(r, col, G, pair, vr, vc);
+          const int ra = ((vr >> 4) <= (vc >> 4)) ? vr : vc;
+          const int cb = ((vr >> 4) <= (vc >> 4)) ? vc : vr;
+          const int first = slot0 + pair * nchunks;
           float sx = 0.f, sy = 0.f;
-          for (int c = slot0; c < slot0 + nchunks; ++c) {
+          for (int c = first; c < first + nchunks; ++c) {
             const cf w = kpart[(((size_t)c * nfpad + k) * E_pad + ra) * E_pad + cb];
             sx += w.x;
             sy += w.y;
           }
           v = {sx, sy};
         }
-        Ks[r][cc] = v;
+        Ks[rr][cc] = v;
       }
       for (int q = tid; q < AP_CH * 2 * MAXT; q += 256) {
         const int cc = q % AP_CH, ti = q / AP_CH;
@@ -814,7 +863,7 @@ __global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, const M
       if (e < p.E) {
         const int cmax = min(AP_CH, p.E - c0);
         for (int cc = 0; cc < cmax; ++cc) {
-          const cf kv = Ks[e][cc];
+          const cf kv = Ks[e - rbase][cc];
 #pragma unroll
           for (int i = 0; i < MAXT; ++i) cmac(acc[i], kv, As[cc][2 * i + tg]);
         }
@@ -835,6 +884,9 @@ __global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, const M
 // ---------------------------------------------------------------- host side
 struct MsmPlan {
   int ET, FBK, nfb, nfpad, threads, grid, n_items;
+  int groups;  // element groups of 64
+  int npair;   // units per scatterer chunk: 1, or one per pair of groups above 2 groups
+  int ept;     // elements per geometry-table row: 64 groups
   bool shallow, table;
   size_t lds, kpart_bytes, scat_offset, rt_offset, la_offset, qt_offset, queue_offset, ws_bytes;
   MsmUnits units;
@@ -871,16 +923,17 @@ static bool msm_fast_mode(const Problem& p) { return !p.fulldir && p.baffle == F
 static bool shallow_required(const Problem& p) { return p.NS > 2 || p.fulldir; }
 
 // The geometry table holds log2 of the amplitude, so it needs amplitudes >= 0:
-// centre-frequency directivity with kappa <= pi and a non-negative obliquity.
+// centre-frequency directivity with kappa <= pi and a non-negative obliquity; and a lane holds the
+// table entries of all its sub-elements, so at most 4 of them.
 static bool table_possible(const Problem& p) {
-  return !p.fulldir && p.kappa <= kPi && !(p.baffle == FS_BAFFLE_IMPEDANCE && !(p.baffle_b >= 0.f));
+  return p.NS <= 4 && !p.fulldir && p.kappa <= kPi && !(p.baffle == FS_BAFFLE_IMPEDANCE && !(p.baffle_b >= 0.f));
 }
 constexpr size_t MSM_TABLE_AUTO_MAX = (size_t)32 << 30;  // bytes of table used without being asked
 
 static std::array<long long, 8> tuning_key(const Problem& p, int P) {
   long long tiles = std::max(1, (p.S + MST - 1) / MST), lg = 0;
   while ((1LL << (lg + 1)) <= tiles) ++lg;
-  return {(long long)current_device(), p.E <= 64 ? 4 : 8, p.NS, p.fulldir ? 2 : (msm_fast_mode(p) ? 0 : 1),
+  return {(long long)current_device(), (p.E + 63) / 64, p.NS, p.fulldir ? 2 : (msm_fast_mode(p) ? 0 : 1),
           p.NF, P, lg, 0};
 }
 
@@ -909,13 +962,14 @@ int msm_tuning_set(const Problem& p, int P, const fs_tuning* t) {
 }
 
 // Scatterer chunks of a single phantom: the count whose last round of
-// equal items is fullest (rounds / chunks smallest), at least 8 tiles each.
-static int pick_chunks(int nfb, int tiles, int slots) {
-  const int cmax = std::max(1, std::min(16, tiles / 8));
+// equal items is fullest (rounds / chunks smallest), at least 8 tiles each;
+// every chunk is npair units (one per pair of element groups).
+static int pick_chunks(int nfb, int tiles, int slots, int npair) {
+  const int cmax = std::max(1, std::min(std::min(16, tiles / 8), MSM_MAX_UNITS / npair));
   double best = 1e30;
   int best_c = 1;
   for (int c = 1; c <= cmax; ++c) {
-    const double rounds = (double)(((long long)nfb * c + slots - 1) / slots);
+    const double rounds = (double)(((long long)nfb * c * npair + slots - 1) / slots);
     const double cost = rounds / c;
     if (cost < best * 0.995) {
       best = cost;
@@ -940,6 +994,14 @@ static int msm_plan_et(const Problem& p, int P, const int64_t* off, const fs_tun
   m.threads = SH::THREADS;
   m.nfb = (p.NF + m.FBK - 1) / m.FBK;
   m.nfpad = m.nfb * m.FBK;
+  m.groups = (p.E + 63) / 64;
+  m.npair = m.groups <= 2 ? 1 : m.groups * (m.groups - 1) / 2;
+  m.ept = 64 * m.groups;
+  if ((long long)P * m.npair > MSM_MAX_UNITS) return FS_ERR_UNSUPPORTED;
+  std::vector<int> pair_code(m.npair, 1 << 16);  // (0, 1) with at most two groups
+  if (m.groups > 2)
+    for (int a = 0; a < m.groups; ++a)
+      for (int b = a + 1; b < m.groups; ++b) pair_code[group_pair_index(a, b, m.groups)] = a | (b << 16);
   const int slots = device_cus() * SH::WG_PER_CU;
   std::vector<int> tiles(P), chunks(P);
   long long total = 0;
@@ -948,31 +1010,33 @@ static int msm_plan_et(const Problem& p, int P, const int64_t* off, const fs_tun
     total += std::max(tiles[q], 1);
   }
   if (P == 1) {
-    chunks[0] = tune.msm_units > 0 ? std::min(tune.msm_units, std::max(tiles[0], 1))
-                                   : pick_chunks(m.nfb, std::max(tiles[0], 1), slots);
+    chunks[0] = tune.msm_units > 0 ? std::min(std::min(tune.msm_units, std::max(tiles[0], 1)), MSM_MAX_UNITS / m.npair)
+                                   : pick_chunks(m.nfb, std::max(tiles[0], 1), slots, m.npair);
   } else {
     const double div = tune.msm_item_div > 0 ? (double)tune.msm_item_div : 8.0;
-    double tmax = std::max(8.0, (double)total * m.nfb / (slots * div));
+    double tmax = std::max(8.0, (double)total * m.nfb * m.npair / (slots * div));
     for (;;) {
       int n = 0;
       for (int q = 0; q < P; ++q) {
         chunks[q] = std::max(1, (int)std::ceil(tiles[q] / tmax));
         n += chunks[q];
       }
-      if (n <= MSM_MAX_UNITS) break;
+      if (n * m.npair <= MSM_MAX_UNITS) break;
       tmax *= 1.25;
     }
   }
   std::vector<MsmUnit> us;
   m.phantoms.n = P;
+  m.phantoms.groups = m.groups;
   for (int q = 0; q < P; ++q) {
     const int per = std::max(1, (tiles[q] + chunks[q] - 1) / chunks[q]) * MST;
     m.phantoms.ph[q] = make_int2((int)us.size(), chunks[q]);
-    for (int c = 0; c < chunks[q]; ++c) {
-      const long long b = std::min<long long>(off[q] + (long long)c * per, off[q + 1]);
-      const long long e = std::min<long long>(b + per, off[q + 1]);
-      us.push_back(MsmUnit{(int)b, (int)e, (int)us.size(), 0});
-    }
+    for (int pi = 0; pi < m.npair; ++pi)
+      for (int c = 0; c < chunks[q]; ++c) {
+        const long long b = std::min<long long>(off[q] + (long long)c * per, off[q + 1]);
+        const long long e = std::min<long long>(b + per, off[q + 1]);
+        us.push_back(MsmUnit{(int)b, (int)e, (int)us.size(), pair_code[pi]});
+      }
   }
   std::stable_sort(us.begin(), us.end(),
                    [](const MsmUnit& a, const MsmUnit& b) { return a.s_end - a.s_begin > b.s_end - b.s_begin; });
@@ -985,7 +1049,7 @@ static int msm_plan_et(const Problem& p, int P, const int64_t* off, const fs_tun
   m.kpart_bytes = (size_t)m.units.n * m.nfpad * ep * ep * sizeof(cf);
   m.scat_offset = (m.kpart_bytes + 255) / 256 * 256;
   m.rt_offset = m.scat_offset + ((size_t)off[P] * sizeof(float4) + 255) / 256 * 256;
-  const size_t terms = (size_t)(off[P] + 1) * ep * p.NS;  // + the dead row
+  const size_t terms = (size_t)(off[P] + 1) * m.ept * p.NS;  // + the dead row
   const size_t table_bytes = terms * (sizeof(double) + 2 * sizeof(float) + sizeof(float2));
   m.table = table_possible(p) && tune.msm_table != 2 && (tune.msm_table == 1 || table_bytes <= MSM_TABLE_AUTO_MAX);
   m.la_offset = m.rt_offset + (m.table ? (terms * sizeof(double) + 255) / 256 * 256 : 0);
@@ -1009,7 +1073,7 @@ static int msm_plan(const Problem& p, int P, const int64_t* off, MsmPlan* m) {
 }
 
 size_t mfma_workspace_batch(const Problem& p, int P, const int64_t* off) {
-  if (p.E > 128 || p.NS > 4) return 256;
+  if (p.E > MSM_MAX_ELEMENTS) return 256;
   MsmPlan m;
   if (msm_plan(p, P, off, &m) != FS_OK) return 0;
   return m.ws_bytes;
@@ -1034,7 +1098,7 @@ static int launch_msm(const Problem& p, const MsmPlan& m, const float4* scat, co
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, queue, 0);
   const int slot = timing_start(st);
   hipLaunchKernelGGL(kern, dim3(m.grid), dim3(m.threads), m.lds, st, p, scat, rt, la, qt, m.units, queue, m.nfpad,
-                     kpart);
+                     m.ept, kpart);
   timing_stop(slot, st);
   return FS_OK;
 }
@@ -1042,6 +1106,11 @@ static int launch_msm(const Problem& p, const MsmPlan& m, const float4* scat, co
 template <int NSUB, int ET>
 static int launch_msm_fd(const Problem& p, const MsmPlan& m, const float4* scat, const double* rt, const float* la,
                          const float2* qt, int* queue, cf* kpart, hipStream_t st) {
+  if constexpr (NSUB == 0) {  // run-time sub-element count: no table, one frequency per wave
+    if (p.fulldir) return launch_msm<0, ET, MSM_FULLDIR, true>(p, m, scat, rt, la, qt, queue, kpart, st);
+    if (msm_fast_mode(p)) return launch_msm<0, ET, MSM_FAST, true>(p, m, scat, rt, la, qt, queue, kpart, st);
+    return launch_msm<0, ET, MSM_GENERIC, true>(p, m, scat, rt, la, qt, queue, kpart, st);
+  } else {
   if (m.table) {
     if constexpr (NSUB > 2) return launch_msm<NSUB, ET, MSM_TABLE, true>(p, m, scat, rt, la, qt, queue, kpart, st);
     else
@@ -1060,14 +1129,17 @@ static int launch_msm_fd(const Problem& p, const MsmPlan& m, const float4* scat,
     return m.shallow ? launch_msm<NSUB, ET, MSM_GENERIC, true>(p, m, scat, rt, la, qt, queue, kpart, st)
                      : launch_msm<NSUB, ET, MSM_GENERIC, false>(p, m, scat, rt, la, qt, queue, kpart, st);
   }
+  }  // NSUB > 0
 }
 
 template <int NSUB>
 static int launch_msm_et(const Problem& p, const MsmPlan& m, const float4* scat, double* rt, float* la, float2* qt,
                          int* queue, cf* kpart, hipStream_t st) {
-  if (m.table && p.S > 0) {
-    const long long n = (long long)(p.S + 1) * m.ET * 16;
-    hipLaunchKernelGGL(k_geo_table<NSUB>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, scat, m.ET * 16, rt,
+  if constexpr (NSUB == 0) {
+    if (m.table) return FS_ERR_UNSUPPORTED;  // table_possible excludes it
+  } else if (m.table && p.S > 0) {
+    const long long n = (long long)(p.S + 1) * m.ept;
+    hipLaunchKernelGGL(k_geo_table<NSUB>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, scat, m.ept, rt,
                        la, qt);
   }
   return m.ET == 4 ? launch_msm_fd<NSUB, 4>(p, m, scat, rt, la, qt, queue, kpart, st)
@@ -1079,7 +1151,7 @@ static int launch_msm_et(const Problem& p, const MsmPlan& m, const float4* scat,
 // so that the call can be captured in a graph.
 int mfma_spectrum_batch(const Problem& p, int P, const int64_t* off, cf* spect, char* ws, size_t ws_bytes,
                         hipStream_t st) {
-  if (p.E > 128) return FS_ERR_UNSUPPORTED;
+  if (p.E > MSM_MAX_ELEMENTS) return FS_ERR_UNSUPPORTED;
   MsmPlan m;
   int rc = msm_plan(p, P, off, &m);
   if (rc != FS_OK) return rc;
@@ -1096,10 +1168,10 @@ int mfma_spectrum_batch(const Problem& p, int P, const int64_t* off, cf* spect, 
     case 2: rc = launch_msm_et<2>(p, m, scat, rt, la, qt, queue, kpart, st); break;
     case 3: rc = launch_msm_et<3>(p, m, scat, rt, la, qt, queue, kpart, st); break;
     case 4: rc = launch_msm_et<4>(p, m, scat, rt, la, qt, queue, kpart, st); break;
-    default: return FS_ERR_UNSUPPORTED;
+    default: rc = launch_msm_et<0>(p, m, scat, rt, la, qt, queue, kpart, st); break;
   }
   if (rc != FS_OK) return rc;
-  const dim3 ag(p.NF, P);
+  const dim3 ag(p.NF, P, (p.E + 127) / 128);
   if (p.NT <= 2)
     hipLaunchKernelGGL(k_msm_apply<1>, ag, dim3(256), 0, st, p, m.ET * 16, m.phantoms, m.nfpad, kpart, spect);
   else if (p.NT <= 16)

@@ -312,9 +312,10 @@ def test_pfield_convex_out_of_field(cuda):
 
 
 # ----------------------------------------------------------------- element counts that do not fill 16-element tiles
-@pytest.mark.parametrize("n_elem", [1, 7, 33, 100, 192])
+@pytest.mark.parametrize("n_elem", [1, 7, 33, 100, 192, 1100])
 def test_ragged_element_counts(n_elem, cuda):
-    """Partial element tiles of both methods; the batched method stops at 128 elements (NotImplementedError)."""
+    """Partial element tiles of both methods; the batched method runs up to 1024 elements (NotImplementedError
+    above)."""
     p = fs.TransducerParams(freq_center=5e6, pitch=3.0e-4, width=2.5e-4, n_elements=n_elem, bandwidth=0.7)
     m = fs.MediumParams(attenuation=0.3)
     x, z, rc = speckle(90 + n_elem, 23, xlim=(-0.01, 0.01), zlim=(0.005, 0.03))
@@ -323,7 +324,7 @@ def test_ragged_element_counts(n_elem, cuda):
     print("n_elem", n_elem, flush=True)
     rf32, _ = _run("hip", x, z, rc, D, p, m)
     assert rel_l2(rf32, want) < FP32_TOL
-    if n_elem > 128:
+    if n_elem > 1024:
         with pytest.raises(NotImplementedError):
             _run("hip_mfma", x, z, rc, D, p, m)
     else:

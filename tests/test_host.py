@@ -303,3 +303,25 @@ def test_adjoint_method_is_a_per_call_argument():
     sizes = [lib.fs_simus_adjoint_workspace_bytes(C.byref(d), m) for m in (0, 1, 2)]
     assert all(s > 0 for s in sizes)
     assert sizes[0] == sizes[1] != sizes[2]  # auto takes the matrix cores at 8 elements, 1 sub-element
+
+
+@pytest.mark.parametrize("n_elem,n_sub", [(64, 1), (128, 2), (192, 2), (200, 3), (130, 6)])
+def test_batched_workspace_layout(n_elem, n_sub):
+    """Workspace of the batched method for a 40-scatterer problem (one scatterer chunk): partial multistatic
+    matrices of 16 ET x 16 ET per unit and padded frequency (one unit, or one per pair of 64-element groups
+    above 128 elements), the scatterer table, the geometry table over elements padded to the 64-element
+    groups (at most 4 sub-elements) and the queue word. Computed on the host, no device needed."""
+    lib = _lib.load()
+    S, NF = 40, 101
+    d = _host_desc(n_scat=S, n_elem=n_elem, n_sub=n_sub, n_freq=NF, n_tx=3)
+    got = lib.fs_simus_workspace_bytes(C.byref(d), _lib.FS_METHOD_MFMA_BF16X3)
+    al = lambda v: (v + 255) // 256 * 256  # noqa: E731
+    groups = -(-n_elem // 64)
+    units = 1 if groups <= 2 else groups * (groups - 1) // 2
+    ep = 64 if n_elem <= 64 else 128
+    fbk = 2 if n_sub > 2 else 4
+    nfpad = -(-NF // fbk) * fbk
+    kpart = units * nfpad * ep * ep * 8
+    terms = (S + 1) * 64 * groups * n_sub
+    table = al(terms * 8) + al(terms * 8) + al(terms * 8) if n_sub <= 4 else 0
+    assert got == al(kpart) + al(S * 16) + table + 256
