@@ -527,6 +527,77 @@ def bench_pfield(args) -> int:
     return 0
 
 
+def bench_beamform(args) -> int:
+    """Consumers of the RF stack: the RF of 75 plane waves over [-16, 16] deg (128-element 7.6 MHz array,
+    20 000 speckle scatterers, simulated once outside the timing) demodulated to IQ and delay-and-sum
+    beamformed onto a 256 x 384 grid (x in [-2, 2] cm, z in [0.5, 4.5] cm), summed over the firings. One step
+    = fs_rf2iq over the stack + fs_das of all firings; both kernels timed with HIP events."""
+    import torch
+
+    import fastsimus_amd as fs
+    from fastsimus_amd import beamform as BF
+    from oracle import beamform_oracle as BO
+
+    dev = torch.device("cuda", 0)
+    params, medium, x, z, rc, delays = workload(n_scat=20_000)
+    res = fs.simus(np.stack([x, z], -1), rc, delays, params, medium, strategy="hip_mfma")
+    rf = res.rf.contiguous()
+    T, N, E = rf.shape
+    fs_hz, fc = 4 * params.freq_center, params.freq_center
+    gx, gz = np.meshgrid(np.linspace(-0.02, 0.02, 256), np.linspace(0.005, 0.045, 384))
+    F = BF.f_number_for(params)
+
+    def step():
+        iq = BF.rf2iq(rf, fs_hz, fc)
+        return iq, BF.das(iq, delays, params, gx, gz, fs_hz, fc, f_number=F)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        iq, img = step()
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    # the two kernels apart (HIP events on torch's current stream, which the library launches on)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record()
+    iq = BF.rf2iq(rf, fs_hz, fc)
+    ev[1].record()
+    img = BF.das(iq, delays, params, gx, gz, fs_hz, fc, f_number=F)
+    ev[2].record()
+    torch.cuda.synchronize()
+    t_iq, t_das = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+    # FP64 / SciPy checkers on samples: 2 firings of IQ, 300 pixels of the image
+    rf_h, iq_h = rf.cpu().numpy(), iq.cpu().numpy()
+    e_iq = max(rel_l2(np.abs(iq_h[t]), np.abs(BO.rf2iq(rf_h[t], fs_hz, fc))) for t in (0, T - 1))
+    pos, _, _ = fs.element_positions(E, params.pitch, params.radius)
+    idx = np.random.default_rng(4).choice(gx.size, 300, replace=False)
+    t0 = time.perf_counter()
+    want = BO.das(iq_h, delays, pos[:, 0], pos[:, 1], gx.reshape(-1)[idx].astype(np.float32),
+                  gz.reshape(-1)[idx].astype(np.float32), fs_hz, fc, 1540.0, F)
+    t_cpu = time.perf_counter() - t0
+    e_das = rel_l2(img.cpu().numpy().reshape(-1)[idx], want)
+    evals = gx.size * E * T
+    out = {
+        "metric": "G(pixel·element·firing)/s, IQ demodulation + delay-and-sum of the RF stack", "value": evals / t_step / 1e9,
+        "unit": "G(pixel·element·firing)/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "precision": "FP32 IIR sections and interpolation; FP64 delays and phase reduction",
+        "data": "synthetic",
+        "config": {"workload": "rf2iq + DAS compound of 75 plane-wave firings (128-element 7.6 MHz array, 20 000 "
+                               "speckle scatterers) onto a 256 x 384 grid", "n_firings": T, "n_samples": N,
+                   "n_elem": E, "n_pixels": int(gx.size), "f_number": F},
+        "kernel_ms": {"fs_rf2iq": t_iq, "fs_das": t_das},
+        "roofline": None,
+        "rel_l2": {"iq_vs_scipy_filtfilt": e_iq, "das_vs_fp64_checker_300_pixels": e_das},
+        "cpu_baseline": {"value": 300 * E * T / t_cpu / 1e9, "unit": "G(pixel·element·firing)/s", "cores": 1,
+                         "kind": "port", "sample": f"300 pixels, FP64 NumPy, {t_cpu:.2f} s"},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -539,7 +610,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint", choices=("auto", "matrix", "vector"), default="auto",
                     help="adjoint15: backward kernels (matrix cores where they apply, or FP32 vector kernels)")
-    ap.add_argument("--workload", choices=("planewave75", "phantoms64", "adjoint15", "pfield", "single"),
+    ap.add_argument("--workload", choices=("planewave75", "phantoms64", "adjoint15", "pfield", "single", "beamform"),
                     default="planewave75",
                     help="planewave75 (default, the headline metric), phantoms64 (uneven phantom batch, phased "
                          "array, focused beams), adjoint15 (forward + gradient of the differentiable form) or "
@@ -583,6 +654,8 @@ def main() -> int:
         return bench_single(args)
     if args.workload == "pfield":
         return bench_pfield(args)
+    if args.workload == "beamform":
+        return bench_beamform(args)
     params, medium, x, z, rc, delays = workload(n_scat=args.n_scat, n_tx=args.n_tx)
     sc = np.stack([x, z], -1)
     plan = fs.simus_precompute(sc, rc, delays, params, medium)

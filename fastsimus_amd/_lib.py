@@ -39,6 +39,9 @@ EXPORTS = (
     "fs_pfield_workspace_bytes",
     "fs_pfield",
     "fs_max_distance",
+    "fs_rf2iq_workspace_bytes",
+    "fs_rf2iq",
+    "fs_das",
     "fs_set_kernel_timing",
     "fs_kernel_time_ms",
 )
@@ -59,6 +62,13 @@ class SimusDesc(C.Structure):
         ("freq_center", C.c_double), ("seg_length", C.c_double), ("radius", C.c_double),
         ("apex_offset", C.c_double), ("baffle_impedance", C.c_double), ("correction", C.c_double),
     ]
+
+
+class Iir(C.Structure):
+    """Mirror of ``fs_iir``: cascaded second-order sections, odd-extension length, section steady states."""
+
+    _fields_ = [("n_sections", C.c_int32), ("padlen", C.c_int32), ("sos", (C.c_double * 6) * 8),
+                ("zi", (C.c_double * 2) * 8)]
 
 
 class Tuning(C.Structure):
@@ -118,6 +128,14 @@ def load() -> C.CDLL:
     lib.fs_pfield.argtypes = [D, P, P, C.c_size_t, P]
     lib.fs_max_distance.restype = C.c_int
     lib.fs_max_distance.argtypes = [P, P, C.c_longlong, P, P, C.c_int, P, P]
+    F = C.POINTER(Iir)
+    lib.fs_rf2iq_workspace_bytes.restype = C.c_size_t
+    lib.fs_rf2iq_workspace_bytes.argtypes = [C.c_int, C.c_int, F]
+    lib.fs_rf2iq.restype = C.c_int
+    lib.fs_rf2iq.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, F, P, P, C.c_size_t, P]
+    lib.fs_das.restype = C.c_int
+    lib.fs_das.argtypes = [P, C.c_int, C.c_int, C.c_int, P, P, P, P, P, C.c_longlong, C.c_double, C.c_double,
+                           C.c_double, C.c_double, C.c_double, C.c_int, P, P]
     lib.fs_set_kernel_timing.restype = C.c_int
     lib.fs_set_kernel_timing.argtypes = [C.c_int]
     lib.fs_kernel_time_ms.restype = C.c_double

@@ -28,6 +28,9 @@
  *   fs_tuning_set / get   <- the fixed tuning constants of kernels/cuda_simus.py:39-45
  *   fs_max_distance       <- the O(points x sources) distance maximum of the plans,
  *                            src/fast_simus/simus.py:219-222 and pfield.py:271-282
+ *   fs_rf2iq, fs_das      <- the consumers of the multi-firing RF stack in the reference's harness
+ *                            (pymust.rf2iq and pymust.dasmtx per firing, summed over firings,
+ *                            tests/_picmus_phantom_shared.py:407-460)
  */
 #ifndef FASTSIMUS_H
 #define FASTSIMUS_H
@@ -200,6 +203,35 @@ int fs_pfield(const fs_simus_desc* d, float* out, void* workspace, size_t worksp
  * the caller reads out[0] when it needs the plan. n_points may be 0 (out = 0). */
 int fs_max_distance(const double* x, const double* z, long long n_points, const double* src_x, const double* src_z,
                     int n_src, double* out, void* stream);
+
+/* IQ demodulation of RF, rf[t][n][e] float32 -> iq[t][n][e] interleaved complex float32:
+ *   iq = 2 LPF(rf(n) exp(-i 2 pi fc n / fs)),
+ * the low-pass run forward and backward (zero phase) over the sequence extended by padlen samples of
+ * its odd reflection at each end, each pass starting from the steady state of its first input value.
+ * The filter is cascaded second-order sections (sos[k] = b0 b1 b2 a0 a1 a2) with zi[k] the section's
+ * steady-state states (transposed direct form II) for a unit constant input to that section.
+ * n_samples must exceed padlen. workspace >= fs_rf2iq_workspace_bytes(...). Replaces pymust.rf2iq
+ * (Butterworth low-pass with filtfilt) as the reference's harness calls it per firing. */
+typedef struct fs_iir {
+  int32_t n_sections; /* 1 .. 8 */
+  int32_t padlen;
+  double sos[8][6];
+  double zi[8][2];
+} fs_iir;
+size_t fs_rf2iq_workspace_bytes(int n_firings, int n_channels, const fs_iir* filter);
+int fs_rf2iq(const float* rf, int n_firings, int n_samples, int n_channels, double fs_hz, double fc_hz,
+             const fs_iir* filter, float* iq, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Delay-and-sum of IQ iq[t][n][e] (interleaved complex float32) onto n_pixels pixels (pixel_x, pixel_z
+ * float32 device arrays): for firing t the transmit arrival is tau_tx = min_e(tx_delay[t][e] + |p - x_e| / c)
+ * (NaN delays: elements that do not transmit), element e samples tau = tau_tx + |p - x_e| / c; the IQ at
+ * (tau - t0) fs is interpolated linearly, rotated by exp(i 2 pi fc tau) and summed over the elements with
+ * |x_p - x_e| <= z_p / (2 f_number) (f_number 0: all elements). compound != 0 writes image[p], the sum over
+ * firings; else image[t][p]. Elements elem_x / elem_z and tx_delay [t][e] are float64 device arrays.
+ * Replaces pymust.dasmtx applied per firing (tests/_picmus_phantom_shared.py:407-460). */
+int fs_das(const float* iq, int n_firings, int n_samples, int n_channels, const double* elem_x, const double* elem_z,
+           const double* tx_delay, const float* pixel_x, const float* pixel_z, long long n_pixels, double fs_hz,
+           double fc_hz, double speed_of_sound, double t0, double f_number, int compound, float* image, void* stream);
 
 /* Optional timing of the dominant kernel launches (k_msm for FS_METHOD_MFMA_BF16X3, k_tx + k_rx for
  * FS_METHOD_FP32): while enabled, each such launch is bracketed by HIP events on its stream.
