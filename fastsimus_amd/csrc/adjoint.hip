@@ -472,9 +472,12 @@ __global__ void k_adjm_conj(Problem p, int nfp, const cf* __restrict__ gspect, c
 }
 
 // Split-BF16 fragments of R for transmit group q, per frequency and element slice w:
-//   R1[k][w][n][hl][lane]: B operand of Y = G R, lane (m = 16 n + l % 16; elements 16 w + 4 (l / 16) + c)
-//   R2[k][w][x][hl][lane]: A operand of W^T = R L^T, lane (element 16 w + l % 16; m = 16 x + 4 (l / 16) + c)
-// each as [Re of the 4 | Im of the 4]; zero for t >= NT, e >= E and k >= NF (k < nfp).
+//   R1[k][w][n][hl][lane]: B operand of Y = G R, lane (m = 16 n + l % 16; elements 16 w + 4 (l / 16) + c),
+//     as [Re of the 4 | Im of the 4];
+//   R2[k][w][x][hl][lane]: A operand of W^T = R L^T, lane (element 16 w + l % 16; m = 16 x + 4 (l / 16) + c),
+//     interleaved: word c = (Re, Im) of entry c (the order of the contraction index inside a fragment is
+//     free as long as both operands share it; the L^T operands are written in the same order);
+// zero for t >= NT, e >= E and k >= NF (k < nfp).
 __global__ void k_adjm_rtab(Problem p, int nfp, int q, int nw, const cf* __restrict__ A, const cf* __restrict__ Gc,
                             am_u32x4* __restrict__ R1, am_u32x4* __restrict__ R2) {
   const long total = (long)nfp * nw * 2 * 64;
@@ -517,10 +520,10 @@ __global__ void k_adjm_rtab(Problem p, int nfp, int q, int nw, const cf* __restr
         re[c] = v.x;
         im[c] = v.y;
       }
-      am_split(re[0], re[1], h0, l0);
-      am_split(re[2], re[3], h1, l1);
-      am_split(im[0], im[1], h2, l2);
-      am_split(im[2], im[3], h3, l3);
+      am_split(re[0], im[0], h0, l0);
+      am_split(re[1], im[1], h1, l1);
+      am_split(re[2], im[2], h2, l2);
+      am_split(re[3], im[3], h3, l3);
       R2[am_ridx(k, nw, w, x, 0, lane)] = (am_u32x4){h0, h1, h2, h3};
       R2[am_ridx(k, nw, w, x, 1, lane)] = (am_u32x4){l0, l1, l2, l3};
     }
@@ -580,8 +583,11 @@ __device__ __forceinline__ void am_mfma1(const AmOps& o, const am_u32x4 (&b1)[2]
 }
 
 // Y of every slice summed in a fixed order; L(s, m ^ 16) = c PP Y(s, m) written as the two
-// split-BF16 B operands of W^T = R L^T: [Re L | -Im L] (real part) and [Im L | Re L] (imaginary part).
-// LDS: ypr / ypi [w][m][s] (rows of AM_YROW floats); lfrag [x][part][hl][lane][8].
+// split-BF16 B operands of W^T = R L^T in the interleaved order of R2: word c of a lane's fragment is
+// (Re L, -Im L) of its entry c (real part) or (Im L, Re L) (imaginary part), so each entry is one
+// 32-bit word per operand: the sign flip of the upper half or the swap of the halves of its split pair.
+// A lane writes 4 words (hi and lo of both parts); the 64 lanes of a wave write 64 consecutive words.
+// LDS: ypr / ypi [w][m][s] (rows of AM_YROW floats); lfrag [x][part][hl][lane][4 words].
 template <int NW>
 __device__ __forceinline__ void am_reduce(const float* ypr, const float* ypi, unsigned short* lfrag, cf cpp) {
 #pragma unroll
@@ -605,18 +611,12 @@ __device__ __forceinline__ void am_reduce(const float* ypr, const float* ypi, un
     const int xx = mt >> 4, mm = mt & 15;
     const int ln = sl + 16 * (mm >> 2), cc = mm & 3;
     unsigned hb, lb;
-    am_split(Lv.x, Lv.y, hb, lb);
-    const unsigned short rh = (unsigned short)(hb & 0xffffu), ih = (unsigned short)(hb >> 16);
-    const unsigned short rl = (unsigned short)(lb & 0xffffu), il = (unsigned short)(lb >> 16);
-    unsigned short* f = lfrag + (xx * 4 * 64 + ln) * 8;   // [x][part 0, hl 0]
-    f[cc] = rh;                                           // part 0 (real), hi
-    f[4 + cc] = (unsigned short)(ih ^ 0x8000u);
-    f[64 * 8 + cc] = rl;                                  // part 0, lo
-    f[64 * 8 + 4 + cc] = (unsigned short)(il ^ 0x8000u);
-    f[2 * 64 * 8 + cc] = ih;                              // part 1 (imaginary), hi
-    f[2 * 64 * 8 + 4 + cc] = rh;
-    f[3 * 64 * 8 + cc] = il;                              // part 1, lo
-    f[3 * 64 * 8 + 4 + cc] = rl;
+    am_split(Lv.x, Lv.y, hb, lb);  // (Re, Im) in the low / high half
+    unsigned* f = (unsigned*)lfrag + (xx * 4 * 64 + ln) * 4 + cc;  // [x][part 0, hl 0][ln][cc]
+    f[0] = hb ^ 0x80000000u;                                       // part 0 (real), hi: (Re, -Im)
+    f[64 * 4] = lb ^ 0x80000000u;                                  // part 0, lo
+    f[2 * 64 * 4] = __builtin_amdgcn_alignbit(hb, hb, 16);         // part 1 (imaginary), hi: (Im, Re)
+    f[3 * 64 * 4] = __builtin_amdgcn_alignbit(lb, lb, 16);         // part 1, lo
   }
 }
 
