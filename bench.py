@@ -548,8 +548,8 @@ def bench_beamform(args) -> int:
     F = BF.f_number_for(params)
 
     def step():
-        iq = BF.rf2iq(rf, fs_hz, fc)
-        return iq, BF.das(iq, delays, params, gx, gz, fs_hz, fc, f_number=F)
+        iq = BF.rf2iq(rf, fs_hz, fc, channel_major=True)
+        return iq, BF.das(iq, delays, params, gx, gz, fs_hz, fc, f_number=F, channel_major=True)
 
     for _ in range(args.warmup):
         step()
@@ -562,14 +562,14 @@ def bench_beamform(args) -> int:
     # the two kernels apart (HIP events on torch's current stream, which the library launches on)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     ev[0].record()
-    iq = BF.rf2iq(rf, fs_hz, fc)
+    iq = BF.rf2iq(rf, fs_hz, fc, channel_major=True)
     ev[1].record()
-    img = BF.das(iq, delays, params, gx, gz, fs_hz, fc, f_number=F)
+    img = BF.das(iq, delays, params, gx, gz, fs_hz, fc, f_number=F, channel_major=True)
     ev[2].record()
     torch.cuda.synchronize()
     t_iq, t_das = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
     # FP64 / SciPy checkers on samples: 2 firings of IQ, 300 pixels of the image
-    rf_h, iq_h = rf.cpu().numpy(), iq.cpu().numpy()
+    rf_h, iq_h = rf.cpu().numpy(), iq.permute(0, 2, 1).cpu().numpy()
     e_iq = max(rel_l2(np.abs(iq_h[t]), np.abs(BO.rf2iq(rf_h[t], fs_hz, fc))) for t in (0, T - 1))
     pos, _, _ = fs.element_positions(E, params.pitch, params.radius)
     idx = np.random.default_rng(4).choice(gx.size, 300, replace=False)
@@ -587,7 +587,8 @@ def bench_beamform(args) -> int:
         "data": "synthetic",
         "config": {"workload": "rf2iq + DAS compound of 75 plane-wave firings (128-element 7.6 MHz array, 20 000 "
                                "speckle scatterers) onto a 256 x 384 grid", "n_firings": T, "n_samples": N,
-                   "n_elem": E, "n_pixels": int(gx.size), "f_number": F},
+                   "n_elem": E, "n_pixels": int(gx.size), "f_number": F,
+                   "iq_layout": "channel-major (firing, element, sample), as beamform() runs it"},
         "kernel_ms": {"fs_rf2iq": t_iq, "fs_das": t_das},
         "roofline": None,
         "rel_l2": {"iq_vs_scipy_filtfilt": e_iq, "das_vs_fp64_checker_300_pixels": e_das},

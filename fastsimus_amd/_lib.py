@@ -132,10 +132,10 @@ def load() -> C.CDLL:
     lib.fs_rf2iq_workspace_bytes.restype = C.c_size_t
     lib.fs_rf2iq_workspace_bytes.argtypes = [C.c_int, C.c_int, F]
     lib.fs_rf2iq.restype = C.c_int
-    lib.fs_rf2iq.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, F, P, P, C.c_size_t, P]
+    lib.fs_rf2iq.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, F, C.c_int, P, P, C.c_size_t, P]
     lib.fs_das.restype = C.c_int
-    lib.fs_das.argtypes = [P, C.c_int, C.c_int, C.c_int, P, P, P, P, P, C.c_longlong, C.c_double, C.c_double,
-                           C.c_double, C.c_double, C.c_double, C.c_int, P, P]
+    lib.fs_das.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P, P, P, C.c_longlong, C.c_double,
+                           C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, P, P]
     lib.fs_set_kernel_timing.restype = C.c_int
     lib.fs_set_kernel_timing.argtypes = [C.c_int]
     lib.fs_kernel_time_ms.restype = C.c_double

@@ -107,8 +107,10 @@ def f_number_for(params: TransducerParams, fc: float | None = None, speed_of_sou
 
 
 # ----------------------------------------------------------------------------- device calls
-def rf2iq(rf, fs: float, fc: float, bandwidth: float | None = None) -> torch.Tensor:
-    """IQ of RF ``(n_samples, n_elements)`` or ``(n_firings, n_samples, n_elements)``: complex64, same shape."""
+def rf2iq(rf, fs: float, fc: float, bandwidth: float | None = None, *, channel_major: bool = False) -> torch.Tensor:
+    """IQ of RF ``(n_samples, n_elements)`` or ``(n_firings, n_samples, n_elements)``: complex64, same shape, or
+    with the last two axes swapped (``channel_major``: ``(n_firings, n_elements, n_samples)``, each sequence
+    contiguous, the layout ``das`` reads fastest)."""
     dev = K.hip_device(rf)
     x = rf.detach() if isinstance(rf, torch.Tensor) else torch.as_tensor(np.asarray(rf))
     x = x.to(device=dev, dtype=torch.float32).contiguous()
@@ -124,10 +126,10 @@ def rf2iq(rf, fs: float, fc: float, bandwidth: float | None = None) -> torch.Ten
     lib = _lib.load()
     import ctypes as C
 
-    out = torch.empty((T, N, E, 2), dtype=torch.float32, device=dev)
+    out = torch.empty(((T, E, N, 2) if channel_major else (T, N, E, 2)), dtype=torch.float32, device=dev)
     ws = K._workspace(lib.fs_rf2iq_workspace_bytes(T, E, C.byref(f)), dev)
-    st = lib.fs_rf2iq(x.data_ptr(), T, N, E, float(fs), float(fc), C.byref(f), out.data_ptr(), ws.data_ptr(),
-                      ws.numel(), K._stream(dev))
+    st = lib.fs_rf2iq(x.data_ptr(), T, N, E, float(fs), float(fc), C.byref(f), 1 if channel_major else 0,
+                      out.data_ptr(), ws.data_ptr(), ws.numel(), K._stream(dev))
     _lib.check(st, "fs_rf2iq")
     iq = torch.view_as_complex(out)
     return iq[0] if squeeze else iq
@@ -135,9 +137,10 @@ def rf2iq(rf, fs: float, fc: float, bandwidth: float | None = None) -> torch.Ten
 
 def das(iq, delays, params: TransducerParams, x, z, fs: float, fc: float | None = None, *,
         speed_of_sound: float = 1540.0, f_number: float | None = None, t0: float = 0.0,
-        compound: bool = True) -> torch.Tensor:
-    """Delay-and-sum of IQ ``(n_firings, n_samples, n_elements)`` (or one firing ``(n_samples, n_elements)``)
-    fired with ``delays`` ``(n_firings, n_elements)`` (or ``(n_elements,)``) onto pixels ``(x, z)`` (arrays
+        compound: bool = True, channel_major: bool = False) -> torch.Tensor:
+    """Delay-and-sum of IQ ``(n_firings, n_samples, n_elements)`` (or one firing ``(n_samples, n_elements)``;
+    ``(n_firings, n_elements, n_samples)`` with ``channel_major``, as ``rf2iq(..., channel_major=True)`` writes
+    it) fired with ``delays`` ``(n_firings, n_elements)`` (or ``(n_elements,)``) onto pixels ``(x, z)`` (arrays
     of one shape). Returns complex64 of the pixels' shape, summed over firings (``compound``), or with a
     leading firing axis."""
     dev = K.hip_device(iq)
@@ -146,7 +149,10 @@ def das(iq, delays, params: TransducerParams, x, z, fs: float, fc: float | None 
     if q.dim() == 2:
         q = q[None]
     q = torch.view_as_real(q.contiguous()).contiguous()
-    T, N, E, _ = q.shape
+    if channel_major:
+        T, E, N, _ = q.shape
+    else:
+        T, N, E, _ = q.shape
     if E != params.n_elements:
         raise ValueError(f"IQ has {E} channels for {params.n_elements} elements")
     d = np.atleast_2d(np.asarray(delays.detach().cpu() if isinstance(delays, torch.Tensor) else delays,
@@ -168,9 +174,9 @@ def das(iq, delays, params: TransducerParams, x, z, fs: float, fc: float | None 
     P = px.numel()
     out = torch.empty(((1 if compound else T), P, 2), dtype=torch.float32, device=dev)
     lib = _lib.load()
-    st = lib.fs_das(q.data_ptr(), T, N, E, ex.data_ptr(), ez.data_ptr(), dd.data_ptr(), px.data_ptr(), pz.data_ptr(),
-                    P, float(fs), fc, float(speed_of_sound), float(t0), F, 1 if compound else 0, out.data_ptr(),
-                    K._stream(dev))
+    st = lib.fs_das(q.data_ptr(), T, N, E, 1 if channel_major else 0, ex.data_ptr(), ez.data_ptr(), dd.data_ptr(),
+                    px.data_ptr(), pz.data_ptr(), P, float(fs), fc, float(speed_of_sound), float(t0), F,
+                    1 if compound else 0, out.data_ptr(), K._stream(dev))
     _lib.check(st, "fs_das")
     img = torch.view_as_complex(out)
     return img[0].reshape(xs.shape) if compound else img.reshape(T, *xs.shape)
@@ -188,8 +194,9 @@ def beamform(rf, delays, params: TransducerParams, x, z, fs: float, fc: float | 
     d = np.atleast_2d(np.asarray(delays.detach().cpu() if isinstance(delays, torch.Tensor) else delays))
     if r.dim() == 3 and r.shape[0] != d.shape[0] and r.shape[2] == d.shape[0]:
         r = r.permute(2, 0, 1)  # rf_stack layout (samples, channels, firings)
-    iq = rf2iq(r, fs, fc, bandwidth)
-    return das(iq, d, params, x, z, fs, fc, speed_of_sound=speed_of_sound, f_number=f_number, t0=t0)
+    iq = rf2iq(r, fs, fc, bandwidth, channel_major=True)
+    return das(iq, d, params, x, z, fs, fc, speed_of_sound=speed_of_sound, f_number=f_number, t0=t0,
+               channel_major=True)
 
 
 __all__ = ["beamform", "butterworth_sos", "das", "f_number_for", "rf2iq", "rf2iq_filter", "section_steady_states"]

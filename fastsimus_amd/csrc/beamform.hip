@@ -32,11 +32,11 @@ struct Cx {
   float x, y;
 };
 
-// One cascade step (transposed direct form II per section) on a complex sample.
+// One cascade step (transposed direct form II per section) on a complex sample; NS sections.
+template <int NS>
 __device__ __forceinline__ Cx iir_step(const IirDev& f, Cx (&s)[kMaxSections][2], Cx v) {
 #pragma unroll
-  for (int k = 0; k < kMaxSections; ++k) {
-    if (k >= f.n_sections) break;
+  for (int k = 0; k < NS; ++k) {
     const float b0 = f.b[k][0], b1 = f.b[k][1], b2 = f.b[k][2], a1 = f.a[k][0], a2 = f.a[k][1];
     const Cx y = {fmaf(b0, v.x, s[k][0].x), fmaf(b0, v.y, s[k][0].y)};
     s[k][0] = {fmaf(b1, v.x, fmaf(-a1, y.x, s[k][1].x)), fmaf(b1, v.y, fmaf(-a1, y.y, s[k][1].y))};
@@ -47,10 +47,10 @@ __device__ __forceinline__ Cx iir_step(const IirDev& f, Cx (&s)[kMaxSections][2]
 }
 
 // States of every section after an infinitely long constant input v.
+template <int NS>
 __device__ __forceinline__ void iir_init(const IirDev& f, Cx (&s)[kMaxSections][2], Cx v) {
 #pragma unroll
-  for (int k = 0; k < kMaxSections; ++k) {
-    if (k >= f.n_sections) break;
+  for (int k = 0; k < NS; ++k) {
     s[k][0] = {f.zi[k][0] * v.x, f.zi[k][0] * v.y};
     s[k][1] = {f.zi[k][1] * v.x, f.zi[k][1] * v.y};
     // the next section sees this one's constant output: v times its DC gain
@@ -80,7 +80,12 @@ __device__ __forceinline__ Cx extended(const float* __restrict__ rf, size_t stri
   return {2.f * last.x - m.x, 2.f * last.y - m.y};
 }
 
-// rf[t][n][e] -> iq[t][n][e] (interleaved complex). tail[seq][pad]: forward output past the end.
+// rf[t][n][e] -> iq[t][n][e] (interleaved complex), or iq[t][e][n] (CM: channel-major, each sequence
+// contiguous, the layout k_das gathers from fastest). tail[seq][pad]: forward output past the end.
+// Both passes take their inputs in blocks of RF_BLOCK samples, loaded before the block's recursion
+// runs, so that one memory latency covers a block instead of every sample.
+constexpr int RF_BLOCK = 16;
+template <int NS, bool CM>
 __global__ __launch_bounds__(256) void k_rf2iq(const float* __restrict__ rf, int T, int N, int E, double cyc,
                                                const IirDev f, float2* __restrict__ iq, float2* __restrict__ tail) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -88,41 +93,68 @@ __global__ __launch_bounds__(256) void k_rf2iq(const float* __restrict__ rf, int
   if (e >= E) return;
   const size_t base = (size_t)t * N * E + e;
   const float* x = rf + base;
-  float2* y = iq + base;
+  float2* y = iq + (CM ? ((size_t)t * E + e) * N : base);
+  const size_t ys = CM ? 1 : (size_t)E;  // stride of the output samples
   float2* tl = tail + ((size_t)t * E + e) * f.padlen;
   const int pad = f.padlen;
+  const int L = N + 2 * pad;  // length of the extended signal
   const Cx first = mixed(x, E, 0, cyc), last = mixed(x, E, N - 1, cyc);
   Cx s[kMaxSections][2];
   // forward pass over the extended signal
-  const Cx e0 = extended(x, E, N, pad, 0, cyc, first, last);
-  iir_init(f, s, e0);
-  Cx v;
-  for (int i = 0; i < N + 2 * pad; ++i) {
-    v = iir_step(f, s, extended(x, E, N, pad, i, cyc, first, last));
-    if (i >= pad && i < pad + N) y[(size_t)(i - pad) * E] = make_float2(v.x, v.y);
-    else if (i >= pad + N) tl[i - pad - N] = make_float2(v.x, v.y);
-  }
-  // backward pass from the end of the extension, starting in the steady state of its first value
-  iir_init(f, s, v);
-  for (int i = N + 2 * pad - 1; i >= pad; --i) {
-    Cx u;
-    if (i >= pad + N) {
-      const float2 w = tl[i - pad - N];
-      u = {w.x, w.y};
-    } else {
-      const float2 w = y[(size_t)(i - pad) * E];
-      u = {w.x, w.y};
+  iir_init<NS>(f, s, extended(x, E, N, pad, 0, cyc, first, last));
+  Cx v = {0.f, 0.f};
+  for (int i0 = 0; i0 < L; i0 += RF_BLOCK) {
+    Cx in[RF_BLOCK];
+#pragma unroll
+    for (int j = 0; j < RF_BLOCK; ++j) in[j] = i0 + j < L ? extended(x, E, N, pad, i0 + j, cyc, first, last) : Cx{0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < RF_BLOCK; ++j) {
+      const int i = i0 + j;
+      if (i < L) {
+        v = iir_step<NS>(f, s, in[j]);
+        if (i >= pad && i < pad + N) y[(size_t)(i - pad) * ys] = make_float2(v.x, v.y);
+        else if (i >= pad + N) tl[i - pad - N] = make_float2(v.x, v.y);
+      }
     }
-    const Cx o = iir_step(f, s, u);
-    if (i < pad + N) y[(size_t)(i - pad) * E] = make_float2(2.f * o.x, 2.f * o.y);
+  }
+  // backward pass from the end of the extension, starting in the steady state of its first value;
+  // every index of a block is read before the block writes any of them
+  iir_init<NS>(f, s, v);
+  for (int i0 = L - 1; i0 >= pad; i0 -= RF_BLOCK) {
+    Cx in[RF_BLOCK];
+#pragma unroll
+    for (int j = 0; j < RF_BLOCK; ++j) {
+      const int i = i0 - j;
+      float2 w = make_float2(0.f, 0.f);
+      if (i >= pad + N) w = tl[i - pad - N];
+      else if (i >= pad) w = y[(size_t)(i - pad) * ys];
+      in[j] = {w.x, w.y};
+    }
+#pragma unroll
+    for (int j = 0; j < RF_BLOCK; ++j) {
+      const int i = i0 - j;
+      if (i >= pad) {
+        const Cx o = iir_step<NS>(f, s, in[j]);
+        if (i < pad + N) y[(size_t)(i - pad) * ys] = make_float2(2.f * o.x, 2.f * o.y);
+      }
+    }
   }
 }
 
 // ---------------------------------------------------------------- delay-and-sum
 // img[p] (compound) or img[t][p]: pixels (px, pz); elements (ex, ez) float64; delays [t][e] float64 (NaN
-// marks an element that does not transmit); iq[t][n][e].
-// Delays are FP64 (a 60 us path in FP32 would carry ~3e-4 rad of phase error at fc); the interpolation
-// weight and the rotation after the phase reduction are FP32.
+// marks an element that does not transmit); iq[t][n][e], or iq[t][e][n] (CM: the two interpolation
+// nodes of a term are adjacent, and the lanes of a wave, neighbouring pixels, read nearby samples of
+// one sequence).
+// One thread per pixel, the firings in groups of DAS_TG: per group, the FP64 distance to every element
+// is evaluated twice (transmit arrival, then receive), not once per firing and element. The sample
+// index (tau - t0) fs and the phase fc tau of a term are sums of a firing part (tau_tx) and an
+// element part (|p - x_e| / c), each split in FP64 into an integer and a fraction (the phase modulo
+// one revolution): the term adds the two fractions in FP32 (error ~1e-7 sample or revolution, against
+// ~3e-4 rad at fc for a 60 us path carried in FP32). The interpolation weight and the rotation are FP32.
+constexpr int DAS_TG = 8;
+constexpr double DAS_INDEX_LIMIT = 1e9;  // sample indices beyond this are out of any record
+template <bool CM>
 __global__ __launch_bounds__(256) void k_das(const float2* __restrict__ iq, int T, int N, int E,
                                              const double* __restrict__ ex, const double* __restrict__ ez,
                                              const double* __restrict__ delay, const float* __restrict__ px,
@@ -133,42 +165,83 @@ __global__ __launch_bounds__(256) void k_das(const float2* __restrict__ iq, int 
   if (p >= P) return;
   const double x = px[p], z = pz[p];
   const float amax = fabsf((float)z) * half_aperture_per_depth;
+  const int i_hi = N - 2;  // last left interpolation node (N >= 2)
   float sx = 0.f, sy = 0.f;
-  for (int t = 0; t < T; ++t) {
-    const double* dt = delay + (size_t)t * E;
-    double ttx = INFINITY;
+  for (int tg = 0; tg < T; tg += DAS_TG) {
+    // transmit arrival of each firing of the group: min over transmitting elements (FP64)
+    double ttx[DAS_TG];
+#pragma unroll
+    for (int j = 0; j < DAS_TG; ++j) ttx[j] = INFINITY;
     for (int e = 0; e < E; ++e) {
-      const double d = dt[e];
-      if (!isfinite(d)) continue;
       const double dx = x - ex[e], dz = z - ez[e];
-      ttx = fmin(ttx, d + sqrt(fma(dx, dx, dz * dz)) * inv_c);
-    }
-    float ax = 0.f, ay = 0.f;
-    if (isfinite(ttx)) {
-      const float2* q = iq + (size_t)t * N * E;
-      for (int e = 0; e < E; ++e) {
-        const double dx = x - ex[e], dz = z - ez[e];
-        if (fabsf((float)dx) > amax) continue;
-        const double tau = ttx + sqrt(fma(dx, dx, dz * dz)) * inv_c;
-        const double si = (tau - t0) * fs;
-        if (!(si >= 0.0) || si > (double)(N - 1)) continue;
-        const int i0 = min((int)si, N - 2 >= 0 ? N - 2 : 0);
-        const float w = (float)(si - (double)i0);
-        const float2 a = q[(size_t)i0 * E + e];
-        const float2 b = N > 1 ? q[(size_t)(i0 + 1) * E + e] : a;
-        const float vr = fmaf(w, b.x - a.x, a.x), vi = fmaf(w, b.y - a.y, a.y);
-        const double ph = fc * tau;
-        const float r = (float)(ph - floor(ph));
-        const float c = __builtin_amdgcn_cosf(r), s = __builtin_amdgcn_sinf(r);
-        ax = fmaf(vr, c, fmaf(-vi, s, ax));
-        ay = fmaf(vr, s, fmaf(vi, c, ay));
+      const double rc = sqrt(fma(dx, dx, dz * dz)) * inv_c;
+#pragma unroll
+      for (int j = 0; j < DAS_TG; ++j) {
+        if (tg + j >= T) break;
+        const double d = delay[(size_t)(tg + j) * E + e];
+        if (isfinite(d)) ttx[j] = fmin(ttx[j], d + rc);
       }
     }
-    if (compound) {
-      sx += ax;
-      sy += ay;
-    } else {
-      img[(size_t)t * P + p] = make_float2(ax, ay);
+    // firing parts: sample index (tau_tx - t0) fs = ia + fa, phase fc tau_tx = integer + pa
+    int ia[DAS_TG];
+    float fa[DAS_TG], pa[DAS_TG];
+    bool live[DAS_TG];
+#pragma unroll
+    for (int j = 0; j < DAS_TG; ++j) {
+      const double A = (ttx[j] - t0) * fs;
+      live[j] = tg + j < T && A > -DAS_INDEX_LIMIT && A < DAS_INDEX_LIMIT;  // false for an infinite tau_tx
+      const double Af = live[j] ? floor(A) : 0.0;
+      ia[j] = (int)Af;
+      fa[j] = live[j] ? (float)(A - Af) : 0.f;
+      const double ph = live[j] ? fc * ttx[j] : 0.0;
+      pa[j] = (float)(ph - floor(ph));
+    }
+    float ax[DAS_TG], ay[DAS_TG];
+#pragma unroll
+    for (int j = 0; j < DAS_TG; ++j) ax[j] = ay[j] = 0.f;
+    for (int e = 0; e < E; ++e) {
+      const double dx = x - ex[e], dz = z - ez[e];
+      if (fabsf((float)dx) > amax) continue;
+      const double rc = sqrt(fma(dx, dx, dz * dz)) * inv_c;
+      const double B = rc * fs;
+      if (!(B < DAS_INDEX_LIMIT)) continue;
+      const double Bf = floor(B);
+      const int ib = (int)Bf;
+      const float fb = (float)(B - Bf);
+      const double phb = fc * rc;
+      const float pb = (float)(phb - floor(phb));
+#pragma unroll
+      for (int j = 0; j < DAS_TG; ++j) {
+        if (!live[j]) continue;
+        // si = (ia + ib) + (fa + fb), fractions in [0, 1)
+        float fr = fa[j] + fb;
+        int i = ia[j] + ib;
+        if (fr >= 1.f) {
+          fr -= 1.f;
+          ++i;
+        }
+        if (i < 0 || i > N - 1 || (i == N - 1 && fr > 0.f)) continue;  // si outside [0, N - 1]
+        const int i0 = min(i, i_hi);
+        const float w = (float)(i - i0) + fr;
+        const float2* q = CM ? iq + ((size_t)(tg + j) * E + e) * N + i0 : iq + ((size_t)(tg + j) * N + i0) * E + e;
+        const float2 a = q[0];
+        const float2 b = q[CM ? 1 : E];  // N >= 2: i0 + 1 <= N - 1
+        const float vr = fmaf(w, b.x - a.x, a.x), vi = fmaf(w, b.y - a.y, a.y);
+        const float r = __builtin_amdgcn_fractf(pa[j] + pb);
+        const float c = __builtin_amdgcn_cosf(r), sn = __builtin_amdgcn_sinf(r);
+        ax[j] = fmaf(vr, c, fmaf(-vi, sn, ax[j]));
+        ay[j] = fmaf(vr, sn, fmaf(vi, c, ay[j]));
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < DAS_TG; ++j) {
+      if (tg + j >= T) break;
+      if (compound) {
+        sx += ax[j];
+        sy += ay[j];
+      } else {
+        img[(size_t)(tg + j) * P + p] = make_float2(ax[j], ay[j]);
+      }
     }
   }
   if (compound) img[p] = make_float2(sx, sy);
@@ -186,7 +259,8 @@ size_t fs_rf2iq_workspace_bytes(int n_firings, int n_channels, const fs_iir* fil
 }
 
 int fs_rf2iq(const float* rf, int n_firings, int n_samples, int n_channels, double fs_hz, double fc_hz,
-             const fs_iir* filter, float* iq, void* workspace, size_t workspace_bytes, void* stream) {
+             const fs_iir* filter, int channel_major, float* iq, void* workspace, size_t workspace_bytes,
+             void* stream) {
   if (!rf || !iq || !filter || n_firings <= 0 || n_channels <= 0 || !(fs_hz > 0) || !(fc_hz >= 0))
     return FS_ERR_INVALID;
   if (filter->n_sections < 1 || filter->n_sections > kMaxSections || filter->padlen < 0) return FS_ERR_INVALID;
@@ -206,24 +280,53 @@ int fs_rf2iq(const float* rf, int n_firings, int n_samples, int n_channels, doub
     f.zi[k][1] = (float)filter->zi[k][1];
   }
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_rf2iq, dim3((n_channels + 255) / 256, n_firings), dim3(256), 0, st, rf, n_firings, n_samples,
-                     n_channels, fc_hz / fs_hz, f, (float2*)iq, (float2*)workspace);
+  const dim3 grid((n_channels + 255) / 256, n_firings);
+  const double cyc = fc_hz / fs_hz;
+  switch (f.n_sections) {
+#define FS_RF2IQ_CASE(K)                                                                                       \
+  case K:                                                                                                      \
+    if (channel_major)                                                                                         \
+      hipLaunchKernelGGL((k_rf2iq<K, true>), grid, dim3(256), 0, st, rf, n_firings, n_samples, n_channels, cyc, \
+                         f, (float2*)iq, (float2*)workspace);                                                  \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_rf2iq<K, false>), grid, dim3(256), 0, st, rf, n_firings, n_samples, n_channels,     \
+                         cyc, f, (float2*)iq, (float2*)workspace);                                             \
+    break;
+    FS_RF2IQ_CASE(1)
+    FS_RF2IQ_CASE(2)
+    FS_RF2IQ_CASE(3)
+    FS_RF2IQ_CASE(4)
+    FS_RF2IQ_CASE(5)
+    FS_RF2IQ_CASE(6)
+    FS_RF2IQ_CASE(7)
+    FS_RF2IQ_CASE(8)
+#undef FS_RF2IQ_CASE
+    default:
+      return FS_ERR_INVALID;
+  }
   return hipGetLastError() == hipSuccess ? FS_OK : FS_ERR_LAUNCH;
 }
 
-int fs_das(const float* iq, int n_firings, int n_samples, int n_channels, const double* elem_x, const double* elem_z,
-           const double* tx_delay, const float* pixel_x, const float* pixel_z, long long n_pixels, double fs_hz,
-           double fc_hz, double speed_of_sound, double t0, double f_number, int compound, float* image, void* stream) {
+int fs_das(const float* iq, int n_firings, int n_samples, int n_channels, int channel_major, const double* elem_x,
+           const double* elem_z, const double* tx_delay, const float* pixel_x, const float* pixel_z, long long n_pixels,
+           double fs_hz, double fc_hz, double speed_of_sound, double t0, double f_number, int compound, float* image,
+           void* stream) {
   if (!iq || !elem_x || !elem_z || !tx_delay || !pixel_x || !pixel_z || !image) return FS_ERR_INVALID;
-  if (n_firings <= 0 || n_samples <= 0 || n_channels <= 0 || n_pixels <= 0) return FS_ERR_INVALID;
+  if (n_firings <= 0 || n_samples < 2 || n_channels <= 0 || n_pixels <= 0) return FS_ERR_INVALID;
   if (!(fs_hz > 0) || !(fc_hz >= 0) || !(speed_of_sound > 0) || !(f_number >= 0)) return FS_ERR_INVALID;
   if (n_pixels > (1LL << 31) - 256) return FS_ERR_UNSUPPORTED;
   const int P = (int)n_pixels;
   // aperture half-width per unit depth: z / (2 F); F = 0 opens the full aperture
   const float hapd = f_number > 0 ? (float)(0.5 / f_number) : INFINITY;
-  hipLaunchKernelGGL(k_das, dim3((P + 255) / 256), dim3(256), 0, (hipStream_t)stream, (const float2*)iq, n_firings,
-                     n_samples, n_channels, elem_x, elem_z, tx_delay, pixel_x, pixel_z, P, fs_hz, fc_hz,
-                     1.0 / speed_of_sound, t0, hapd, compound ? 1 : 0, (float2*)image);
+  const dim3 grid((P + 255) / 256);
+  if (channel_major)
+    hipLaunchKernelGGL(k_das<true>, grid, dim3(256), 0, (hipStream_t)stream, (const float2*)iq, n_firings, n_samples,
+                       n_channels, elem_x, elem_z, tx_delay, pixel_x, pixel_z, P, fs_hz, fc_hz, 1.0 / speed_of_sound,
+                       t0, hapd, compound ? 1 : 0, (float2*)image);
+  else
+    hipLaunchKernelGGL(k_das<false>, grid, dim3(256), 0, (hipStream_t)stream, (const float2*)iq, n_firings, n_samples,
+                       n_channels, elem_x, elem_z, tx_delay, pixel_x, pixel_z, P, fs_hz, fc_hz, 1.0 / speed_of_sound,
+                       t0, hapd, compound ? 1 : 0, (float2*)image);
   return hipGetLastError() == hipSuccess ? FS_OK : FS_ERR_LAUNCH;
 }
 

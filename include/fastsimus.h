@@ -204,7 +204,8 @@ int fs_pfield(const fs_simus_desc* d, float* out, void* workspace, size_t worksp
 int fs_max_distance(const double* x, const double* z, long long n_points, const double* src_x, const double* src_z,
                     int n_src, double* out, void* stream);
 
-/* IQ demodulation of RF, rf[t][n][e] float32 -> iq[t][n][e] interleaved complex float32:
+/* IQ demodulation of RF, rf[t][n][e] float32 -> interleaved complex float32 iq[t][n][e] (channel_major 0)
+ * or iq[t][e][n] (channel_major != 0: each sequence contiguous, the fastest input of fs_das):
  *   iq = 2 LPF(rf(n) exp(-i 2 pi fc n / fs)),
  * the low-pass run forward and backward (zero phase) over the sequence extended by padlen samples of
  * its odd reflection at each end, each pass starting from the steady state of its first input value.
@@ -220,18 +221,22 @@ typedef struct fs_iir {
 } fs_iir;
 size_t fs_rf2iq_workspace_bytes(int n_firings, int n_channels, const fs_iir* filter);
 int fs_rf2iq(const float* rf, int n_firings, int n_samples, int n_channels, double fs_hz, double fc_hz,
-             const fs_iir* filter, float* iq, void* workspace, size_t workspace_bytes, void* stream);
+             const fs_iir* filter, int channel_major, float* iq, void* workspace, size_t workspace_bytes,
+             void* stream);
 
-/* Delay-and-sum of IQ iq[t][n][e] (interleaved complex float32) onto n_pixels pixels (pixel_x, pixel_z
- * float32 device arrays): for firing t the transmit arrival is tau_tx = min_e(tx_delay[t][e] + |p - x_e| / c)
- * (NaN delays: elements that do not transmit), element e samples tau = tau_tx + |p - x_e| / c; the IQ at
+/* Delay-and-sum of IQ iq[t][n][e] (interleaved complex float32; iq[t][e][n] when channel_major != 0) onto
+ * n_pixels pixels (pixel_x, pixel_z float32 device arrays): for firing t the transmit arrival is
+ * tau_tx = min_e(tx_delay[t][e] + |p - x_e| / c) (NaN delays: elements that do not transmit), element e
+ * samples tau = tau_tx + |p - x_e| / c; the IQ at
  * (tau - t0) fs is interpolated linearly, rotated by exp(i 2 pi fc tau) and summed over the elements with
  * |x_p - x_e| <= z_p / (2 f_number) (f_number 0: all elements). compound != 0 writes image[p], the sum over
- * firings; else image[t][p]. Elements elem_x / elem_z and tx_delay [t][e] are float64 device arrays.
+ * firings; else image[t][p]. Elements elem_x / elem_z and tx_delay [t][e] are float64 device arrays;
+ * n_samples >= 2.
  * Replaces pymust.dasmtx applied per firing (tests/_picmus_phantom_shared.py:407-460). */
-int fs_das(const float* iq, int n_firings, int n_samples, int n_channels, const double* elem_x, const double* elem_z,
-           const double* tx_delay, const float* pixel_x, const float* pixel_z, long long n_pixels, double fs_hz,
-           double fc_hz, double speed_of_sound, double t0, double f_number, int compound, float* image, void* stream);
+int fs_das(const float* iq, int n_firings, int n_samples, int n_channels, int channel_major, const double* elem_x,
+           const double* elem_z, const double* tx_delay, const float* pixel_x, const float* pixel_z, long long n_pixels,
+           double fs_hz, double fc_hz, double speed_of_sound, double t0, double f_number, int compound, float* image,
+           void* stream);
 
 /* Optional timing of the dominant kernel launches (k_msm for FS_METHOD_MFMA_BF16X3, k_tx + k_rx for
  * FS_METHOD_FP32): while enabled, each such launch is bracketed by HIP events on its stream.

@@ -171,6 +171,26 @@ def test_das_matches_checker_and_compounds(cuda):
 
 
 @pytest.mark.gpu
+def test_channel_major_layout_matches_sample_major(cuda):
+    """rf2iq's channel-major output is its sample-major output transposed, bit for bit (same arithmetic per
+    sequence), and das reads either layout to the same image (same terms; the two layouts only change
+    where the interpolation nodes are read from). Includes firings whose delays have NaN entries."""
+    p, D, rf = _simus_rf(cuda)
+    fs_hz, fc = 4 * p.freq_center, p.freq_center
+    iq = BF.rf2iq(rf, fs_hz, fc)
+    iqc = BF.rf2iq(rf, fs_hz, fc, channel_major=True)
+    assert iqc.shape == (iq.shape[0], iq.shape[2], iq.shape[1])
+    assert bool((iqc.permute(0, 2, 1) == iq).all())
+    D = D.copy()
+    D[1, 60:70] = np.nan
+    gx, gz = np.meshgrid(np.linspace(-0.012, 0.012, 37), np.linspace(0.002, 0.03, 29))
+    for compound in (True, False):
+        a = BF.das(iq, D, p, gx, gz, fs_hz, fc, f_number=1.5, compound=compound).cpu().numpy()
+        b = BF.das(iqc, D, p, gx, gz, fs_hz, fc, f_number=1.5, compound=compound, channel_major=True).cpu().numpy()
+        assert bool(np.array_equal(a, b)), compound
+
+
+@pytest.mark.gpu
 def test_das_point_echo_closed_form(cuda):
     p = fs.L11_5v()
     fs_hz, fc = 4 * p.freq_center, p.freq_center
