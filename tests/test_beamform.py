@@ -171,6 +171,27 @@ def test_das_matches_checker_and_compounds(cuda):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n", [19, 20, 37, 100, 1111, 4099])
+@pytest.mark.parametrize("wn", [0.5, 0.15])
+def test_rf2iq_lengths_and_cutoffs_match_scipy(cuda, n, wn):
+    """Sequence lengths from padlen + 1 (most of the 32 time chunks of a pass empty) to long ones that
+    are not multiples of the chunk count, and a narrower low-pass (poles nearer z = 1, so the chunk
+    transitions carry more of the state), against SciPy's filtfilt recipe."""
+    import torch
+
+    fs_hz, fc = 4e7, 5e6
+    rng = np.random.default_rng(n)
+    rf = rng.standard_normal((2, n, 3))
+    bw = None if wn == 0.5 else wn * fs_hz
+    iq = BF.rf2iq(torch.as_tensor(rf, dtype=torch.float32, device=cuda), fs_hz, fc, bandwidth=bw).cpu().numpy()
+    for t in range(2):
+        want = BO.rf2iq(rf[t].astype(np.float32), fs_hz, fc, bandwidth=bw)
+        e = rel_l2(iq[t], want)
+        print(f"n={n} wn={wn} firing {t}: rel-L2 {e:.2e}", flush=True)
+        assert e <= 1e-5, (n, wn, t, e)
+
+
+@pytest.mark.gpu
 def test_channel_major_layout_matches_sample_major(cuda):
     """rf2iq's channel-major output is its sample-major output transposed, bit for bit (same arithmetic per
     sequence), and das reads either layout to the same image (same terms; the two layouts only change
