@@ -13,9 +13,13 @@ Extensions over the reference signature, all opt-in:
 * ``strategy`` selects the device method: ``SimusStrategy.HIP`` (plain FP32,
   one transmit at a time; also what ``CUDA`` maps to) or
   ``SimusStrategy.HIP_MFMA`` (split-BF16 matrix-core path that shares one
-  multistatic matrix per frequency between all transmits). The reference's
-  host strategies ``PYTHON`` / ``SCAN`` / ``METAL`` do not exist on this
-  device and raise ``NotImplementedError``.
+  multistatic matrix per frequency between all transmits). ``None`` (the
+  default) takes the matrix-core path from ``AUTO_MFMA_MIN_TX`` transmits on
+  (its cost hardly depends on the transmit count, the plain path's grows
+  with it; with fewer transmits the plain path is faster) for arrays of up
+  to 1024 elements, else the plain path. The reference's host strategies
+  ``PYTHON`` / ``SCAN`` / ``METAL`` do not exist on this device and raise
+  ``NotImplementedError``.
 
 Intentional divergence: ``scatterers`` of shape ``(*batch, 2)`` with more
 than one batch axis are flattened correctly (the reference only handles
@@ -55,9 +59,13 @@ class SimusResult(NamedTuple):
     spectrum: torch.Tensor
 
 
-def _method(strategy) -> str:
+AUTO_MFMA_MIN_TX = 3      # transmits from which strategy=None takes the matrix-core path
+AUTO_MFMA_MAX_ELEMENTS = 1024
+
+
+def _method(strategy, n_tx: int = 1, n_elements: int = 0) -> str:
     if strategy is None:
-        return "fp32"
+        return "mfma" if n_tx >= AUTO_MFMA_MIN_TX and n_elements <= AUTO_MFMA_MAX_ELEMENTS else "fp32"
     s = SimusStrategy(strategy)
     if s in (SimusStrategy.HIP, SimusStrategy.CUDA):
         return "fp32"
@@ -74,7 +82,8 @@ def simus_compute(scatterers, rc, delays, plan: SimusPlan, params: TransducerPar
                   medium: MediumParams = _DEFAULT_MEDIUM, *, tx_apodization=None,
                   full_frequency_directivity: bool = False, strategy: SimusStrategy | str | None = None) -> SimusResult:
     """RF and full spectrum for a precomputed plan (``simus.py:392-495``)."""
-    method = _method(strategy)
+    n_tx = int(delays.shape[0]) if _ndim(delays) == 2 else 1
+    method = _method(strategy, n_tx, params.n_elements)
     dev = K.hip_device(scatterers, rc, delays, tx_apodization)
     prob = K.Problem(scatterers, rc, delays, tx_apodization, params, medium, n_sub=plan.n_sub,
                      seg_length=plan.seg_length, freqs=plan.selected_freqs, pulse=plan.pulse_spectrum,

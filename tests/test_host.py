@@ -177,6 +177,10 @@ def test_strategy_names():
     from fastsimus_amd.simus import _method
 
     assert _method(None) == "fp32" and _method("cuda") == "fp32" and _method("hip_mfma") == "mfma"
+    # default: the matrix-core path from 3 transmits on, for arrays it takes
+    assert _method(None, 3, 128) == "mfma" and _method(None, 75, 1024) == "mfma"
+    assert _method(None, 2, 128) == "fp32" and _method(None, 75, 1025) == "fp32"
+    assert _method("hip", 75, 128) == "fp32"
     for s in ("python", "scan", "metal"):
         with pytest.raises(NotImplementedError):
             _method(s)
