@@ -420,7 +420,8 @@ __global__ void k_ppc(Problem p, int nfp, cf* __restrict__ ppc) {
 // 16 w + 4 (l / 16) + c, c < 4: the A-operand layout of G in Y = G R and the
 // result layout of W^T = R L^T, so that G, E_j and W of one term sit in the
 // same lane. Y is summed over the waves through LDS.
-constexpr int AM_FC = 32;    // frequencies per work item = the FP64 reseed interval of the recurrence (3 k + 2)
+constexpr int AM_FC = 32;    // frequencies per chunk = the FP64 reseed interval of the recurrence (3 k + 2)
+constexpr int AM_CPI = 4;    // chunks per work item (the per-item geometry and gradient coefficients are shared)
 constexpr int AM_TG = 16;    // transmits per group
 constexpr int AM_MAXW = 8;   // element slices per workgroup (E <= 128)
 constexpr int AM_NSUB_MAX = 2;
@@ -639,7 +640,7 @@ struct AmG {
 // latencies of the other stages. E, G of three consecutive frequencies rotate through three
 // register sets (the step sequence is unrolled by three), so nothing is copied.
 template <int NSUB, int NW>
-__global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, const am_u32x4* __restrict__ R1,
+__global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, const am_u32x4* __restrict__ R1,
                                                   const am_u32x4* __restrict__ R2, const cf* __restrict__ ppc,
                                                   float* __restrict__ prc, float* __restrict__ px,
                                                   float* __restrict__ pz, float* __restrict__ pc) {
@@ -654,17 +655,16 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, const am_u32
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4, col = lane & 15;
-  const int nch = gridDim.y;
-  const int k0 = blockIdx.y * AM_FC;
+  int k0 = 0;                                 // first frequency of the chunk being run
   const int klast = nch * AM_FC - 1;          // last padded frequency (tables hold nch * AM_FC)
   const int s = blockIdx.x * 16 + col;
   const bool valid = s < p.S;
   const float xs = valid ? p.x[s] : 0.f;
   const float zs = valid ? p.z[s] : 1e-2f;
   const bool in_field = valid && !out_of_field(p, xs, zs);
-  const double fstart = p.f0 + (double)k0 * p.df;
 
-  // E_j(f) = exp(f r_c (-alpha' + i 2 pi / c)) by FP32 recurrence from an FP64-phase seed at fstart
+  // E_j(f) = exp(f r_c (-alpha' + i 2 pi / c)) by FP32 recurrence from an FP64-phase seed at the start
+  // of every chunk; the amplitude and the per-step factor hold for all chunks
   cf EA[4][NSUB], EB[4][NSUB], EC[4][NSUB], Q[4][NSUB];
   float amp[4][NSUB];
 #pragma unroll
@@ -674,9 +674,8 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, const am_u32
     for (int j = 0; j < NSUB; ++j) {
       const GeoF geo = geometry_fast(p, xs, zs, e < p.E ? e : p.E - 1, j);
       amp[c][j] = (in_field && e < p.E) ? geo.amp : 0.f;
-      EC[c][j] = green_fast(geo, fstart, 1.0f);
       Q[c][j] = green_step_fast(geo, p.df);
-      EA[c][j] = EB[c][j] = {0.f, 0.f};
+      EA[c][j] = EB[c][j] = EC[c][j] = {0.f, 0.f};
     }
   }
   AmG GA, GB, GC;
@@ -687,13 +686,6 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, const am_u32
     for (int j = 0; j < NSUB; ++j) S0[c][j] = S1[c][j] = {0.f, 0.f};
   float racc = 0.f;
   am_u32x4 b1[2][2], a2[2][2];
-#pragma unroll
-  for (int x = 0; x < 2; ++x)
-#pragma unroll
-    for (int hl = 0; hl < 2; ++hl) {
-      b1[x][hl] = R1[am_ridx(k0, nw, w, x, hl, lane)];
-      a2[x][hl] = R2[am_ridx(k0, nw, w, x, hl, lane)];
-    }
 
   // stage A(i): operands and Y of f_i from E(f_i); stored to Y buffer i % 2
   auto stage_a = [&](int i, const cf (&Ei)[4][NSUB], AmG& Gi) {
@@ -779,6 +771,26 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, const am_u32
     __syncthreads();
   };
 
+#pragma unroll 1
+  for (int cc = 0; cc < AM_CPI; ++cc) {
+  const int chunk = blockIdx.y * AM_CPI + cc;
+  if (chunk >= nch) break;  // uniform over the workgroup
+  k0 = chunk * AM_FC;
+  const double fstart = p.f0 + (double)k0 * p.df;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int e = 16 * w + 4 * g + c;
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j)
+      EC[c][j] = green_fast(geometry_fast(p, xs, zs, e < p.E ? e : p.E - 1, j), fstart, 1.0f);
+  }
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int hl = 0; hl < 2; ++hl) {
+      b1[x][hl] = R1[am_ridx(k0, nw, w, x, hl, lane)];
+      a2[x][hl] = R2[am_ridx(k0, nw, w, x, hl, lane)];
+    }
   // step 0: E(f_0) in EC, E(f_1) -> EA
   stage_a(0, EC, GC);
   advance(EC, EA);
@@ -799,6 +811,8 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, const am_u32
   stage_c(AM_FC - 2, EC, GC);
   __syncthreads();
   stage_c(AM_FC - 1, EA, GA);
+  __syncthreads();  // the next chunk's first steps rewrite the LDS buffers
+  }  // chunks
 
   // derivative coefficients once per item; sums over the lane's terms, its 4 lane groups and the slices
   float gx = 0.f, gz = 0.f, gcs = 0.f;
@@ -838,7 +852,7 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, const am_u32
       for (int i = 0; i < 4; ++i) a[i] += red[u][i][threadIdx.x];
     const float rcs = p.rc[s];
     const bool live = in_field && rcs != 0.f;
-    const size_t o = ((size_t)q * nch + blockIdx.y) * p.S + s;
+    const size_t o = ((size_t)q * gridDim.y + blockIdx.y) * p.S + s;
     prc[o] = a[0];
     px[o] = live ? a[1] * rcs : 0.f;
     pz[o] = live ? a[2] * rcs : 0.f;
@@ -933,7 +947,7 @@ static int adjoint_vector(const Problem& p, const cf* dspect, float* d_rc, float
 
 // Matrix-core form: workspace [A | Gc | ppc | R1 | R2 | partials rc, x, z, c | dc segments].
 struct AdjmLayout {
-  int nfp, nw, ngrp, nch;
+  int nfp, nw, ngrp, nch, nitem;
   size_t oA, oGc, oppc, oR1, oR2, orc, ox, oz, oc, oseg, total;
 };
 
@@ -941,11 +955,12 @@ static AdjmLayout adjm_layout(const Problem& p) {
   AdjmLayout L{};
   L.nch = (p.NF + AM_FC - 1) / AM_FC;
   L.nfp = L.nch * AM_FC;
+  L.nitem = (L.nch + AM_CPI - 1) / AM_CPI;
   L.nw = p.E <= 64 ? 4 : AM_MAXW;  // element slices (waves) per workgroup: k_adjm<NSUB, 4 or 8>
   L.ngrp = (p.NT + AM_TG - 1) / AM_TG;
   const size_t tab = (size_t)p.NT * p.E * L.nfp * sizeof(cf);
   const size_t rt = (size_t)L.nfp * L.nw * 2 * 2 * 64 * sizeof(am_u32x4);
-  const size_t part = (size_t)L.ngrp * L.nch * p.S * sizeof(float);
+  const size_t part = (size_t)L.ngrp * L.nitem * p.S * sizeof(float);
   size_t o = 0;
   L.oA = o;   o = al(o + tab);
   L.oGc = o;  o = al(o + tab);
@@ -982,21 +997,21 @@ static int adjoint_matrix(const Problem& p, const cf* dspect, float* d_rc, float
   }
   const long nr = (long)L.nfp * L.nw * 2 * 64;
   const int rblocks = (int)std::min<long>((nr + 255) / 256, 65536);
-  const dim3 grid((p.S + 15) / 16, L.nch);
+  const dim3 grid((p.S + 15) / 16, L.nitem);
   for (int q = 0; q < L.ngrp; ++q) {
     hipLaunchKernelGGL(k_adjm_rtab, dim3(rblocks), dim3(256), 0, st, p, L.nfp, q, L.nw, (const cf*)A,
                        (const cf*)Gc, R1, R2);
     const dim3 block(64 * L.nw);
     if (p.NS == 1 && L.nw == 4)
-      hipLaunchKernelGGL((k_adjm<1, 4>), grid, block, 0, st, p, q, R1, R2, ppc, prc, px, pz, pc);
+      hipLaunchKernelGGL((k_adjm<1, 4>), grid, block, 0, st, p, q, L.nch, R1, R2, ppc, prc, px, pz, pc);
     else if (p.NS == 1)
-      hipLaunchKernelGGL((k_adjm<1, 8>), grid, block, 0, st, p, q, R1, R2, ppc, prc, px, pz, pc);
+      hipLaunchKernelGGL((k_adjm<1, 8>), grid, block, 0, st, p, q, L.nch, R1, R2, ppc, prc, px, pz, pc);
     else if (L.nw == 4)
-      hipLaunchKernelGGL((k_adjm<2, 4>), grid, block, 0, st, p, q, R1, R2, ppc, prc, px, pz, pc);
+      hipLaunchKernelGGL((k_adjm<2, 4>), grid, block, 0, st, p, q, L.nch, R1, R2, ppc, prc, px, pz, pc);
     else
-      hipLaunchKernelGGL((k_adjm<2, 8>), grid, block, 0, st, p, q, R1, R2, ppc, prc, px, pz, pc);
+      hipLaunchKernelGGL((k_adjm<2, 8>), grid, block, 0, st, p, q, L.nch, R1, R2, ppc, prc, px, pz, pc);
   }
-  const int nb = L.ngrp * L.nch;
+  const int nb = L.ngrp * L.nitem;
   const dim3 g1((p.S + 255) / 256);
   hipLaunchKernelGGL(k_sum_parts, g1, dim3(256), 0, st, prc, nb, p.S, 0, d_rc);
   hipLaunchKernelGGL(k_sum_parts, g1, dim3(256), 0, st, px, nb, p.S, 0, d_x);
