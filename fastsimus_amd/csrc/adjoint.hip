@@ -436,15 +436,6 @@ typedef float am_floatx4 __attribute__((ext_vector_type(4)));
 typedef float am_floatx2 __attribute__((ext_vector_type(2)));
 typedef unsigned am_u32x4 __attribute__((ext_vector_type(4)));
 
-// (a, b) -> hi = bf16(a) | bf16(b) << 16, lo = the same of the remainders
-__device__ __forceinline__ void am_split(float a, float b, unsigned& hi, unsigned& lo) {
-  const am_bf16x2 h = __builtin_convertvector((am_floatx2){a, b}, am_bf16x2);
-  const unsigned hb = __builtin_bit_cast(unsigned, h);
-  const float ah = __uint_as_float(hb << 16), bh = __uint_as_float(hb & 0xffff0000u);
-  const am_bf16x2 l = __builtin_convertvector((am_floatx2){a - ah, b - bh}, am_bf16x2);
-  hi = hb;
-  lo = __builtin_bit_cast(unsigned, l);
-}
 
 __device__ __forceinline__ am_floatx4 am_mma(am_u32x4 a, am_u32x4 b, am_floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(am_bf16x8, a), __builtin_bit_cast(am_bf16x8, b),
@@ -503,10 +494,10 @@ __global__ void k_adjm_rtab(Problem p, int nfp, int q, int nw, const cf* __restr
         re[c] = v.x;
         im[c] = v.y;
       }
-      am_split(re[0], re[1], h0, l0);
-      am_split(re[2], re[3], h1, l1);
-      am_split(im[0], im[1], h2, l2);
-      am_split(im[2], im[3], h3, l3);
+      split_bf16_pair(re[0], re[1], h0, l0);
+      split_bf16_pair(re[2], re[3], h1, l1);
+      split_bf16_pair(im[0], im[1], h2, l2);
+      split_bf16_pair(im[2], im[3], h3, l3);
       R1[am_ridx(k, nw, w, x, 0, lane)] = (am_u32x4){h0, h1, h2, h3};
       R1[am_ridx(k, nw, w, x, 1, lane)] = (am_u32x4){l0, l1, l2, l3};
     }
@@ -521,10 +512,10 @@ __global__ void k_adjm_rtab(Problem p, int nfp, int q, int nw, const cf* __restr
         re[c] = v.x;
         im[c] = v.y;
       }
-      am_split(re[0], im[0], h0, l0);
-      am_split(re[1], im[1], h1, l1);
-      am_split(re[2], im[2], h2, l2);
-      am_split(re[3], im[3], h3, l3);
+      split_bf16_pair(re[0], im[0], h0, l0);
+      split_bf16_pair(re[1], im[1], h1, l1);
+      split_bf16_pair(re[2], im[2], h2, l2);
+      split_bf16_pair(re[3], im[3], h3, l3);
       R2[am_ridx(k, nw, w, x, 0, lane)] = (am_u32x4){h0, h1, h2, h3};
       R2[am_ridx(k, nw, w, x, 1, lane)] = (am_u32x4){l0, l1, l2, l3};
     }
@@ -552,10 +543,10 @@ __device__ __forceinline__ void am_gen(const float (&amp)[4][NSUB], const cf (&E
   }
   const am_u32x4 nim = {0u, 0u, 0x80008000u, 0x80008000u};
   unsigned h0, l0, h1, l1, h2, l2, h3, l3;
-  am_split(o.gr[0], o.gr[1], h0, l0);
-  am_split(o.gr[2], o.gr[3], h1, l1);
-  am_split(o.gi[0], o.gi[1], h2, l2);
-  am_split(o.gi[2], o.gi[3], h3, l3);
+  split_bf16_pair(o.gr[0], o.gr[1], h0, l0);
+  split_bf16_pair(o.gr[2], o.gr[3], h1, l1);
+  split_bf16_pair(o.gi[0], o.gi[1], h2, l2);
+  split_bf16_pair(o.gi[2], o.gi[3], h3, l3);
   o.a1h = (am_u32x4){h0, h1, h2, h3} ^ nim;
   o.a1l = (am_u32x4){l0, l1, l2, l3} ^ nim;
   o.a2h = (am_u32x4){h2, h3, h0, h1};
@@ -612,7 +603,7 @@ __device__ __forceinline__ void am_reduce(const float* ypr, const float* ypi, un
     const int xx = mt >> 4, mm = mt & 15;
     const int ln = sl + 16 * (mm >> 2), cc = mm & 3;
     unsigned hb, lb;
-    am_split(Lv.x, Lv.y, hb, lb);  // (Re, Im) in the low / high half
+    split_bf16_pair(Lv.x, Lv.y, hb, lb);  // (Re, Im) in the low / high half
     unsigned* f = (unsigned*)lfrag + (xx * 4 * 64 + ln) * 4 + cc;  // [x][part 0, hl 0][ln][cc]
     f[0] = hb ^ 0x80000000u;                                       // part 0 (real), hi: (Re, -Im)
     f[64 * 4] = lb ^ 0x80000000u;                                  // part 0, lo

@@ -144,6 +144,18 @@ __device__ __forceinline__ cf green_step_fast(const GeoF& g, double df) {
   return {m * __builtin_amdgcn_cosf(ph), m * __builtin_amdgcn_sinf(ph)};
 }
 
+// Split-bf16 form of a pair of FP32 values: hi = bf16(a) | bf16(b) << 16 (round to nearest even) and
+// lo = the same of the remainders a - hi_a, b - hi_b (exact in FP32).
+typedef __bf16 fs_bf16x2 __attribute__((ext_vector_type(2)));
+typedef float fs_floatx2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split_bf16_pair(float a, float b, unsigned& hi, unsigned& lo) {
+  const fs_bf16x2 h = __builtin_convertvector((fs_floatx2){a, b}, fs_bf16x2);
+  const unsigned hb = __builtin_bit_cast(unsigned, h);
+  const float ah = __uint_as_float(hb << 16), bh = __uint_as_float(hb & 0xffff0000u);
+  hi = hb;
+  lo = __builtin_bit_cast(unsigned, __builtin_convertvector((fs_floatx2){a - ah, b - bh}, fs_bf16x2));
+}
+
 int timing_start(hipStream_t st);
 void timing_stop(int slot, hipStream_t st);
 

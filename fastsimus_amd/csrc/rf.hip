@@ -80,15 +80,6 @@ constexpr int IT_N = 8;  // row tiles (16 output samples each) per workgroup
 constexpr int kMaxGridZ = 65535;  // transmits per launch (grid z)
 constexpr int IK = 16;   // bins per K chunk (32 real products per MFMA)
 
-// a, b -> packed bf16 hi (round to nearest) and the bf16 of the remainders
-__device__ __forceinline__ void rf_split(float a, float b, unsigned& hi, unsigned& lo) {
-  const rf_bf16x2 h = __builtin_convertvector((rf_floatx2){a, b}, rf_bf16x2);
-  const unsigned hb = __builtin_bit_cast(unsigned, h);
-  const float ah = __uint_as_float(hb << 16), bh = __uint_as_float(hb & 0xffff0000u);
-  const rf_bf16x2 l = __builtin_convertvector((rf_floatx2){a - ah, b - bh}, rf_bf16x2);
-  hi = hb;
-  lo = __builtin_bit_cast(unsigned, l);
-}
 
 __device__ __forceinline__ rf_floatx4 rf_mma(rf_u32x4 a, rf_u32x4 b, rf_floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(rf_bf16x8, a), __builtin_bit_cast(rf_bf16x8, b), c,
@@ -123,7 +114,7 @@ __global__ void k_idft_table(int NF, int idx0, int N, int nkeep, int ntp, int nk
   }
   unsigned h[4], l[4];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) rf_split(v[2 * q], v[2 * q + 1], h[q], l[q]);
+  for (int q = 0; q < 4; ++q) split_bf16_pair(v[2 * q], v[2 * q + 1], h[q], l[q]);
   W[(r * 2 + 0) * 64 + lane] = (rf_u32x4){h[0], h[1], h[2], h[3]};
   W[(r * 2 + 1) * 64 + lane] = (rf_u32x4){l[0], l[1], l[2], l[3]};
 }
@@ -177,7 +168,7 @@ __global__ __launch_bounds__(64 * NW) void k_idft_mfma(const cf* __restrict__ sp
     }
     unsigned h[4], l[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) rf_split(y[2 * q], y[2 * q + 1], h[q], l[q]);
+    for (int q = 0; q < 4; ++q) split_bf16_pair(y[2 * q], y[2 * q + 1], h[q], l[q]);
     const rf_u32x4 bh = {h[0], h[1], h[2], h[3]}, bl = {l[0], l[1], l[2], l[3]};
 #pragma unroll
     for (int i = 0; i < IT_N; ++i) {

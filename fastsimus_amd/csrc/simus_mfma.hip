@@ -71,14 +71,6 @@ struct FragLds {
   }
 };
 
-__device__ __forceinline__ void split_pair(float a, float b, unsigned& hi, unsigned& lo) {
-  const bf16x2 h = __builtin_convertvector((floatx2){a, b}, bf16x2);
-  const unsigned hb = __builtin_bit_cast(unsigned, h);
-  const float ah = __uint_as_float(hb << 16), bh = __uint_as_float(hb & 0xffff0000u);
-  const bf16x2 l = __builtin_convertvector((floatx2){a - ah, b - bh}, bf16x2);
-  hi = hb;
-  lo = __builtin_bit_cast(unsigned, l);
-}
 
 __device__ __forceinline__ floatx4 mma(u32x4 a, u32x4 b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
@@ -401,10 +393,10 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<N
 #pragma unroll
   for (int k = 0; k < FBK; ++k) {
     unsigned h0, h1, h2, h3, l0, l1, l2, l3;
-    split_pair(re[k][0], re[k][1], h0, l0);
-    split_pair(re[k][2], re[k][3], h1, l1);
-    split_pair(im[k][0], im[k][1], h2, l2);
-    split_pair(im[k][2], im[k][3], h3, l3);
+    split_bf16_pair(re[k][0], re[k][1], h0, l0);
+    split_bf16_pair(re[k][2], re[k][3], h1, l1);
+    split_bf16_pair(im[k][0], im[k][1], h2, l2);
+    split_bf16_pair(im[k][2], im[k][3], h3, l3);
     frag[L::idx(k, 0, tile, lane)] = (u32x4){h0, h1, h2, h3};
     frag[L::idx(k, 1, tile, lane)] = (u32x4){l0, l1, l2, l3};
   }
@@ -521,10 +513,10 @@ __device__ __forceinline__ void gen_fragments_table(const BlockScalars& bs, cons
 #pragma unroll
   for (int k = 0; k < FBK; ++k) {
     unsigned h0, h1, h2, h3, l0, l1, l2, l3;
-    split_pair(re[k][0], re[k][1], h0, l0);
-    split_pair(re[k][2], re[k][3], h1, l1);
-    split_pair(im[k][0], im[k][1], h2, l2);
-    split_pair(im[k][2], im[k][3], h3, l3);
+    split_bf16_pair(re[k][0], re[k][1], h0, l0);
+    split_bf16_pair(re[k][2], re[k][3], h1, l1);
+    split_bf16_pair(im[k][0], im[k][1], h2, l2);
+    split_bf16_pair(im[k][2], im[k][3], h3, l3);
     frag[L::idx(k, 0, tile, lane)] = (u32x4){h0, h1, h2, h3};
     frag[L::idx(k, 1, tile, lane)] = (u32x4){l0, l1, l2, l3};
   }
