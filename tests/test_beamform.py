@@ -212,6 +212,36 @@ def test_channel_major_layout_matches_sample_major(cuda):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("f_number,t0", [(0.0, 0.0), (1.0, 2.5e-6), (2.0, -1e-6)])
+def test_das_random_iq_options_match_checker(cuda, f_number, t0):
+    """Random IQ, diverging-wave and focused firings (some elements silent: NaN delays), a grid that
+    reaches past the end of the record and above the array, the full aperture (F = 0) and record start
+    times t0 other than 0, in both IQ layouts, against the FP64 checker."""
+    import torch
+
+    p = fs.L11_5v()
+    fs_hz, fc = 4 * p.freq_center, p.freq_center
+    pos = fs.element_positions(128, p.pitch, p.radius)[0]
+    D = np.stack([fs.focused(pos, np.array([0.004, -0.01]), speed_of_sound=1540.0),
+                  fs.focused(pos, np.array([-0.003, 0.015]), speed_of_sound=1540.0),
+                  fs.plane_wave(pos, np.deg2rad(5.0), speed_of_sound=1540.0)])
+    D[2, :20] = np.nan
+    rng = np.random.default_rng(7)
+    iq_h = rng.standard_normal((3, 700, 128)) + 1j * rng.standard_normal((3, 700, 128))
+    iq = torch.as_tensor(iq_h.astype(np.complex64), device=cuda)
+    gx, gz = np.meshgrid(np.linspace(-0.015, 0.015, 23), np.linspace(-0.002, 0.03, 19))
+    want = BO.das(iq_h.astype(np.complex64), D, pos[:, 0], pos[:, 1], gx.astype(np.float32), gz.astype(np.float32),
+                  fs_hz, fc, 1540.0, f_number, t0=t0).reshape(gx.shape)
+    img = BF.das(iq, D, p, gx, gz, fs_hz, fc, f_number=f_number, t0=t0).cpu().numpy()
+    e = rel_l2(img, want)
+    print(f"F={f_number} t0={t0}: das vs checker rel-L2 {e:.2e}", flush=True)
+    assert e <= 1e-5
+    img_c = BF.das(iq.permute(0, 2, 1).contiguous(), D, p, gx, gz, fs_hz, fc, f_number=f_number, t0=t0,
+                   channel_major=True).cpu().numpy()
+    assert bool(np.array_equal(img, img_c))
+
+
+@pytest.mark.gpu
 def test_das_point_echo_closed_form(cuda):
     p = fs.L11_5v()
     fs_hz, fc = 4 * p.freq_center, p.freq_center
