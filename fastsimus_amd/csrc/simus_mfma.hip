@@ -788,11 +788,49 @@ __global__ void k_set_int(int* __restrict__ ptr, int value) {
   if (threadIdx.x == 0) *ptr = value;
 }
 
+// ---------------------------------------------------------------- fold the scatterer chunks
+// Sums the partial multistatic matrices of every (phantom, pair of element groups) over its scatterer
+// chunks, in chunk order, into the first chunk's slot, and mirrors each off-diagonal upper tile into
+// the lower triangle (k_msm computes the upper tiles, the diagonal ones whole), so that k_msm_apply
+// reads every matrix whole, row by row, from one slot. One workgroup per (frequency, phantom, pair),
+// one 16 x 16 tile at a time through LDS, so that both the direct and the mirrored writes are
+// row-contiguous.
+__global__ __launch_bounds__(256) void k_msm_fold(int E_pad, int nfpad, const MsmPhantoms phantoms,
+                                                  cf* __restrict__ kpart) {
+  __shared__ float2 tile[16][17];
+  const int k = blockIdx.x, ph = blockIdx.y, pair = blockIdx.z;
+  const int nch = phantoms.ph[ph].y;
+  if (nch == 0) return;  // k_msm_apply reads no slot for such a phantom
+  const int first = phantoms.ph[ph].x + pair * nch;
+  const int ET = E_pad / 16;
+  const int rr = threadIdx.x >> 4, cc = threadIdx.x & 15;
+  cf* K0 = kpart + ((size_t)first * nfpad + k) * E_pad * E_pad;
+  for (int a = 0; a < ET; ++a) {
+    for (int b = a; b < ET; ++b) {
+      const size_t o = (size_t)(16 * a + rr) * E_pad + 16 * b + cc;
+      float sx = 0.f, sy = 0.f;
+      for (int c = 0; c < nch; ++c) {
+        const cf w = K0[(size_t)c * nfpad * E_pad * E_pad + o];
+        sx += w.x;
+        sy += w.y;
+      }
+      K0[o] = {sx, sy};
+      if (a != b) {
+        tile[rr][cc] = make_float2(sx, sy);
+        __syncthreads();
+        const float2 t = tile[cc][rr];  // element (16 b + rr, 16 a + cc) of the mirrored tile
+        K0[(size_t)(16 * b + rr) * E_pad + 16 * a + cc] = {t.x, t.y};
+        __syncthreads();
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- apply all transmits
-// out[t][k][e] = corr probe_k PP_k sum_e' K_k[e][e'] A_t(k, e'),  K_k = sum_c Kpart[c][k] (upper tiles, symmetric)
+// out[t][k][e] = corr probe_k PP_k sum_e' K_k[e][e'] A_t(k, e'), K_k whole in the first chunk's slot (k_msm_fold)
 constexpr int AP_CH = 32;  // e' per LDS chunk
 
-// Grid (frequency, phantom, block of 128 elements); phantom ph sums its K slots and writes
+// Grid (frequency, phantom, block of 128 elements); phantom ph reads its folded K and writes
 // out[ph][t][k][e] for the block's elements. MAXT transmits per thread and pass (2 MAXT per pass over
 // the two thread parities): 40, or fewer when the problem has fewer transmits, so that the per-thread
 // accumulation does no padded work.
@@ -820,19 +858,11 @@ __global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, const M
         const int rr = q / AP_CH, cc = q % AP_CH;
         const int r = rbase + rr, col = c0 + cc;
         cf v = {0.f, 0.f};
-        if (r < p.E && col < p.E) {
+        if (r < p.E && col < p.E && nchunks > 0) {
           int pair, vr, vc;
           k_location(r, col, G, pair, vr, vc);
-          const int ra = ((vr >> 4) <= (vc >> 4)) ? vr : vc;
-          const int cb = ((vr >> 4) <= (vc >> 4)) ? vc : vr;
           const int first = slot0 + pair * nchunks;
-          float sx = 0.f, sy = 0.f;
-          for (int c = first; c < first + nchunks; ++c) {
-            const cf w = kpart[(((size_t)c * nfpad + k) * E_pad + ra) * E_pad + cb];
-            sx += w.x;
-            sy += w.y;
-          }
-          v = {sx, sy};
+          v = kpart[(((size_t)first * nfpad + k) * E_pad + vr) * E_pad + vc];
         }
         Ks[rr][cc] = v;
       }
@@ -1163,6 +1193,7 @@ int mfma_spectrum_batch(const Problem& p, int P, const int64_t* off, cf* spect, 
     default: rc = launch_msm_et<0>(p, m, scat, rt, la, qt, queue, kpart, st); break;
   }
   if (rc != FS_OK) return rc;
+  hipLaunchKernelGGL(k_msm_fold, dim3(p.NF, P, m.npair), dim3(256), 0, st, m.ET * 16, m.nfpad, m.phantoms, kpart);
   const dim3 ag(p.NF, P, (p.E + 127) / 128);
   if (p.NT <= 2)
     hipLaunchKernelGGL(k_msm_apply<1>, ag, dim3(256), 0, st, p, m.ET * 16, m.phantoms, m.nfpad, kpart, spect);
