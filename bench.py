@@ -76,13 +76,14 @@ def batched_useful_flops(n_scat: int, n_elem: int, n_freq: int, n_sub: int, n_tx
 
 
 def msm_flops_per_launch(n_scat: int, n_elem: int, n_freq: int, fbk: int = 4) -> float:
-    """MFMA FLOPs issued by one k_msm launch (upper-triangular 16x16 tiles, 6 MFMAs each per 16 scatterers
-    and frequency, frequencies padded to a multiple of the block depth fbk)."""
+    """MFMA FLOPs issued by one k_msm launch (upper-triangular 16x16 tiles per 16 scatterers and frequency:
+    6 MFMAs for each off-diagonal tile, 4 for each diagonal one; frequencies padded to a multiple of the
+    block depth fbk)."""
     et = 4 if n_elem <= 64 else 8
-    pairs = et * (et + 1) // 2
+    mfma = 6 * (et * (et - 1) // 2) + 4 * et
     nfpad = -(-n_freq // fbk) * fbk
     tiles = -(-n_scat // 16)
-    return tiles * nfpad * pairs * 6 * (16 * 16 * 32 * 2)
+    return tiles * nfpad * mfma * (16 * 16 * 32 * 2)
 
 
 TRAFFIC_PROFILE = ROOT / "profiles" / "k_msm_planewave75_200k_traffic.json"

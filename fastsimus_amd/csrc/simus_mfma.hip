@@ -11,7 +11,8 @@
 // so the scatterer sum is done once for all transmits. K_f is a rank-S
 // product H^T H with H = sqrt(rc) G (see below for rc < 0), evaluated with
 // v_mfma_f32_16x16x32_bf16 in split precision: every operand is hi + lo
-// bf16 and hi*hi + hi*lo + lo*hi is accumulated in FP32.
+// bf16 and hi*hi + hi*lo + lo*hi is accumulated in FP32 (on the diagonal tiles,
+// which are symmetric, hi*hi + 2 hi*lo and the symmetric part of the sum).
 //
 // Complex -> real: the 32-deep MFMA contraction index runs over
 // (real part, imaginary part) of 16 scatterers; one fragment of element e
@@ -71,6 +72,14 @@ struct FragLds {
   }
 };
 
+
+// Twice each of the 8 bf16 in a fragment: one added to every exponent field (bit 7 of each 16-bit
+// half). Exact for normal values; a zero becomes 2^-126 and a subnormal lo part at most 2^-125, both
+// far below the FP32 resolution of any sum they enter. No carry crosses the halves: that would need
+// an exponent field of 255 with the sign set (-inf or NaN), which no Green's-function value has.
+__device__ __forceinline__ u32x4 twice_bf16x8(u32x4 v) {
+  return v + (u32x4){0x00800080u, 0x00800080u, 0x00800080u, 0x00800080u};
+}
 
 __device__ __forceinline__ floatx4 mma(u32x4 a, u32x4 b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
@@ -583,10 +592,19 @@ struct WaveTiles {
           floatx4 kr = acc[kk][j][0], ki = acc[kk][j][1];
           kr = mma(arh[r], bh, kr);
           ki = mma(aih[r], bh, ki);
-          kr = mma(arh[r], bl, kr);
-          ki = mma(aih[r], bl, ki);
-          kr = mma(arl[r], bh, kr);
-          ki = mma(ail[r], bh, ki);
+          if (r == c) {
+            // Diagonal tile: with C = hi lo^T, the split product is D + C + C^T (D = hi hi^T and the
+            // tile are symmetric). Accumulating D + 2 C, whose symmetric part k_msm_fold takes, needs
+            // two products instead of three.
+            const u32x4 bl2 = twice_bf16x8(bl);
+            kr = mma(arh[r], bl2, kr);
+            ki = mma(aih[r], bl2, ki);
+          } else {
+            kr = mma(arh[r], bl, kr);
+            ki = mma(aih[r], bl, ki);
+            kr = mma(arl[r], bh, kr);
+            ki = mma(ail[r], bh, ki);
+          }
           acc[kk][j][0] = kr;
           acc[kk][j][1] = ki;
         }
@@ -790,11 +808,11 @@ __global__ void k_set_int(int* __restrict__ ptr, int value) {
 
 // ---------------------------------------------------------------- fold the scatterer chunks
 // Sums the partial multistatic matrices of every (phantom, pair of element groups) over its scatterer
-// chunks, in chunk order, into the first chunk's slot, and mirrors each off-diagonal upper tile into
-// the lower triangle (k_msm computes the upper tiles, the diagonal ones whole), so that k_msm_apply
-// reads every matrix whole, row by row, from one slot. One workgroup per (frequency, phantom, pair),
-// one 16 x 16 tile at a time through LDS, so that both the direct and the mirrored writes are
-// row-contiguous.
+// chunks, in chunk order, into the first chunk's slot, mirrors each off-diagonal upper tile into
+// the lower triangle and replaces each diagonal tile A (k_msm accumulates hi hi^T + 2 hi lo^T there)
+// by its symmetric part (A + A^T) / 2, so that k_msm_apply reads every matrix whole, row by row, from
+// one slot. One workgroup per (frequency, phantom, pair), one 16 x 16 tile at a time through LDS, so
+// that both the direct and the mirrored writes are row-contiguous.
 __global__ __launch_bounds__(256) void k_msm_fold(int E_pad, int nfpad, const MsmPhantoms phantoms,
                                                   cf* __restrict__ kpart) {
   __shared__ float2 tile[16][17];
@@ -814,14 +832,16 @@ __global__ __launch_bounds__(256) void k_msm_fold(int E_pad, int nfpad, const Ms
         sx += w.x;
         sy += w.y;
       }
-      K0[o] = {sx, sy};
+      tile[rr][cc] = make_float2(sx, sy);
+      __syncthreads();
+      const float2 t = tile[cc][rr];  // element (16 b + rr, 16 a + cc) of the mirrored tile
       if (a != b) {
-        tile[rr][cc] = make_float2(sx, sy);
-        __syncthreads();
-        const float2 t = tile[cc][rr];  // element (16 b + rr, 16 a + cc) of the mirrored tile
+        K0[o] = {sx, sy};
         K0[(size_t)(16 * b + rr) * E_pad + 16 * a + cc] = {t.x, t.y};
-        __syncthreads();
+      } else {
+        K0[o] = {0.5f * (sx + t.x), 0.5f * (sy + t.y)};  // t = A^T: exactly symmetric (x + y = y + x)
       }
+      __syncthreads();
     }
   }
 }
