@@ -404,7 +404,7 @@ __global__ void k_ppc(Problem p, int nfp, cf* __restrict__ ppc) {
 //   L(s, m) = c PP Y(s, m ^ 16)     (so that W = L R^T is the W of k_adj_pos summed over t)
 //   W(s, e) = sum_m L(s, m) R(e, m) = sum_t c PP A_t(e) B_t(s) + c P_t(s) conj(g_t(e))
 // and sum_e G(s, e) W(s, e) = 2 sum_t c P_t(s) B_t(s), so that
-//   dL/drc_s  = 1/2 Re sum_{f,e} G W
+//   dL/drc_s  = 1/2 Re sum_{f,e} G W = 1/2 sum_{e,j} amp_j Re S0_j   (G = sum_j amp_j E_j)
 //   dL/dth_s  = rc_s sum_{e,j} a_th Re S0_j + Re(b_th S1_j),
 //   S0_j = sum_f E_j W, S1_j = sum_f f E_j W      (as in k_adj_pos)
 // Y and W^T are split-BF16 MFMA products (hi*hi + hi*lo + lo*hi, FP32
@@ -612,11 +612,6 @@ __device__ __forceinline__ void am_reduce(const float* ypr, const float* ypi, un
   }
 }
 
-// G(f) of the lane's 4 elements (kept for the rc gradient two stages later)
-struct AmG {
-  float gr[4], gi[4];
-};
-
 // One work item: scatterers [16 blockIdx.x, +16), the AM_FC frequencies [AM_FC blockIdx.y, +AM_FC)
 // (the tables are zero past the band, so a padded frequency adds nothing), transmit group q, NW
 // element slices (NW waves, NW * 16 >= E). Writes the item's partial sums (one per scatterer) of
@@ -669,24 +664,17 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
       EA[c][j] = EB[c][j] = EC[c][j] = {0.f, 0.f};
     }
   }
-  AmG GA, GB, GC;
   cf S0[4][NSUB], S1[4][NSUB];
 #pragma unroll
   for (int c = 0; c < 4; ++c)
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) S0[c][j] = S1[c][j] = {0.f, 0.f};
-  float racc = 0.f;
   am_u32x4 b1[2][2], a2[2][2];
 
   // stage A(i): operands and Y of f_i from E(f_i); stored to Y buffer i % 2
-  auto stage_a = [&](int i, const cf (&Ei)[4][NSUB], AmG& Gi) {
+  auto stage_a = [&](int i, const cf (&Ei)[4][NSUB]) {
     AmOps o;
     am_gen<NSUB>(amp, Ei, o);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      Gi.gr[c] = o.gr[c];
-      Gi.gi[c] = o.gi[c];
-    }
     am_floatx4 yr[2], yi[2];
     am_mfma1(o, b1, yr, yi);
     const int kn = min(k0 + i + 1, klast);
@@ -710,7 +698,7 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
   };
   // stage C(i): W^T of f_i (A = R as stored, B = the two variants of L^T; result lane = the lane's
   // own terms) and the terms of f_i
-  auto stage_c = [&](int i, const cf (&Ei)[4][NSUB], const AmG& Gi) {
+  auto stage_c = [&](int i, const cf (&Ei)[4][NSUB]) {
     const am_u32x4* lf4 = (const am_u32x4*)(lfrag + (i & 1) * LB);
     am_floatx4 wr = {0.f, 0.f, 0.f, 0.f}, wi = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -735,8 +723,6 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const cf Wc = {wr[c], wi[c]};
-      racc = fmaf(Gi.gr[c], Wc.x, racc);
-      racc = fmaf(-Gi.gi[c], Wc.y, racc);
 #pragma unroll
       for (int j = 0; j < NSUB; ++j) {
         const cf ew = cmul(Ei[c][j], Wc);
@@ -753,11 +739,11 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
 #pragma unroll
       for (int j = 0; j < NSUB; ++j) Eo[c][j] = cmul(Ei[c][j], Q[c][j]);
   };
-  // steady step i (2 <= i < AM_FC): Ea = E(f_{i-2}) (then E(f_{i+1})), Ec = E(f_i); Ga = G(f_{i-2}), Gc <- G(f_i)
-  auto steady = [&](int i, cf (&Ea)[4][NSUB], cf (&Ec)[4][NSUB], const AmG& Ga, AmG& Gc) {
-    stage_a(i, Ec, Gc);
+  // steady step i (2 <= i < AM_FC): Ea = E(f_{i-2}) (then E(f_{i+1})), Ec = E(f_i)
+  auto steady = [&](int i, cf (&Ea)[4][NSUB], cf (&Ec)[4][NSUB]) {
+    stage_a(i, Ec);
     stage_b(i - 1);
-    stage_c(i - 2, Ea, Ga);
+    stage_c(i - 2, Ea);
     advance(Ec, Ea);
     __syncthreads();
   };
@@ -783,29 +769,36 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
       a2[x][hl] = R2[am_ridx(k0, nw, w, x, hl, lane)];
     }
   // step 0: E(f_0) in EC, E(f_1) -> EA
-  stage_a(0, EC, GC);
+  stage_a(0, EC);
   advance(EC, EA);
   __syncthreads();
   // step 1: E(f_1) in EA, E(f_2) -> EB
-  stage_a(1, EA, GA);
+  stage_a(1, EA);
   stage_b(0);
   advance(EA, EB);
   __syncthreads();
   // steps 2 .. AM_FC - 1, three at a time: (E(f_{i-2}), E(f_i)) = (EC, EB), (EA, EC), (EB, EA)
   for (int i = 2; i < AM_FC; i += 3) {
-    steady(i, EC, EB, GC, GB);
-    steady(i + 1, EA, EC, GA, GC);
-    steady(i + 2, EB, EA, GB, GA);
+    steady(i, EC, EB);
+    steady(i + 1, EA, EC);
+    steady(i + 2, EB, EA);
   }
   // drain: step AM_FC (B of the last frequency, C of the one before), step AM_FC + 1 (C of the last)
   stage_b(AM_FC - 1);
-  stage_c(AM_FC - 2, EC, GC);
+  stage_c(AM_FC - 2, EC);
   __syncthreads();
-  stage_c(AM_FC - 1, EA, GA);
+  stage_c(AM_FC - 1, EA);
   __syncthreads();  // the next chunk's first steps rewrite the LDS buffers
   }  // chunks
 
-  // derivative coefficients once per item; sums over the lane's terms, its 4 lane groups and the slices
+  // derivative coefficients once per item; sums over the lane's terms, its 4 lane groups and the slices.
+  // The amplitudes do not depend on frequency, so sum_f Re(G W) = sum_j amp_j Re S0_j: the rc gradient
+  // comes from the same sums (amp = 0 for padded elements and out-of-field scatterers).
+  float racc = 0.f;
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) racc = fmaf(amp[c][j], S0[c][j].x, racc);
   float gx = 0.f, gz = 0.f, gcs = 0.f;
   if (in_field) {
     const float fc_hz = (float)(p.min_dist > 0 ? 0.5 / (p.min_dist * p.inv_c) : 1.0);
