@@ -12,9 +12,11 @@
 //    each (s, e, j) term is reseeded from an FP64 phase f * r_c / c mod 1 and
 //    advanced by an FP32 complex recurrence inside the block;
 //  * k_tx: lane = scatterer, elements are walked uniformly by the wave, so the
-//    transmit table A(f, e) is a wave-uniform (scalar) load; a thread takes one
-//    or two frequency blocks, whose 32 P(f, s) accumulators each live in VGPRs;
-//    the result Q = rc PP P goes to HBM as Q[block][s][32];
+//    transmit table A(f, e) is a wave-uniform (scalar) load; A is geometric in f,
+//    so (without per-frequency directivity) it is read at each block start and
+//    folded into the recurrence step of every term; a thread takes one or two
+//    frequency blocks, whose 32 P(f, s) accumulators each live in VGPRs; the
+//    result Q = rc PP P goes to HBM as Q[block][s][32];
 //  * k_rx: lane = element, scatterers are walked uniformly, so Q(f, s) is a
 //    scalar load; the 32 X(f, e) accumulators live in VGPRs; every wave writes a
 //    private partial sum (no atomics) and k_reduce adds them in a fixed order,
@@ -101,8 +103,18 @@ __global__ __launch_bounds__(256) void k_tx(Problem p, int tx, int nfp, int fb0,
     for (int k = 0; k < FB; ++k) P[b][k] = {0.f, 0.f};
 
   const cf* At = A + (size_t)tx * p.E * nfp + (size_t)(fb0 + lb0) * FB;
+  const double* taus = p.delay + (size_t)tx * p.E;  // A(f, e) = w_e exp(i 2 pi f tau_e)
   const int ngroups = n_groups<NSUB, GROUPED>(p);
-  for (int e = 0; e < p.E; ++e)
+  for (int e = 0; e < p.E; ++e) {
+  // Without per-frequency directivity the transmit weight is geometric in f as well: h_j = g_j A
+  // advances by q_j exp(i 2 pi df tau_e) (phase reduced in FP64), seeded with A at the block start,
+  // so the inner step is the recurrence and the sum of h_j into P, with no multiply by A.
+  cf astep = {1.f, 0.f};
+  if constexpr (!FULLDIR) {
+    const double nu = p.df * taus[e];
+    const float ph = (float)(nu - floor(nu));
+    astep = {__builtin_amdgcn_cosf(ph), __builtin_amdgcn_sinf(ph)};
+  }
   for (int jg = 0; jg < ngroups; ++jg) {
     GeoF geo[NSUB];
     cf q[NSUB];
@@ -110,33 +122,45 @@ __global__ __launch_bounds__(256) void k_tx(Problem p, int tx, int nfp, int fb0,
     for (int j = 0; j < NSUB; ++j) {
       geo[j] = group_geometry<GROUPED>(p, xs, zs, e, jg * NSUB + j);
       q[j] = green_step_fast(geo[j], p.df);
+      if constexpr (!FULLDIR) q[j] = cmul(q[j], astep);
     }
 #pragma unroll
     for (int b = 0; b < BPT; ++b) {
       if (lb0 + b >= nb) break;  // uniform over the launch row
       const double fstart = p.f0 + (double)((fb0 + lb0 + b) * FB) * p.df;
       cf g[NSUB];
-#pragma unroll
-      for (int j = 0; j < NSUB; ++j) g[j] = green_fast(geo[j], fstart, geo[j].amp);
       const cf* Ae = At + (size_t)e * nfp + b * FB;
+      if constexpr (FULLDIR) {
 #pragma unroll
-      for (int k = 0; k < FB; ++k) {
-        cf G = {0.f, 0.f};
+        for (int j = 0; j < NSUB; ++j) g[j] = green_fast(geo[j], fstart, geo[j].amp);
 #pragma unroll
-        for (int j = 0; j < NSUB; ++j) {
-          if constexpr (FULLDIR) {
+        for (int k = 0; k < FB; ++k) {
+          cf G = {0.f, 0.f};
+#pragma unroll
+          for (int j = 0; j < NSUB; ++j) {
             const float d = sinc_fast(p.kappa_per_hz * (float)(fstart + (double)k * p.df) * geo[j].sint);
             G.x = fmaf(g[j].x, d, G.x);
             G.y = fmaf(g[j].y, d, G.y);
-          } else {
-            G.x += g[j].x;
-            G.y += g[j].y;
+            g[j] = cmul(g[j], q[j]);
           }
-          g[j] = cmul(g[j], q[j]);
+          cmac(P[b][k], G, Ae[k]);
         }
-        cmac(P[b][k], G, Ae[k]);
+      } else {
+        const cf a0 = Ae[0];
+#pragma unroll
+        for (int j = 0; j < NSUB; ++j) g[j] = cmul(green_fast(geo[j], fstart, geo[j].amp), a0);
+#pragma unroll
+        for (int k = 0; k < FB; ++k) {
+#pragma unroll
+          for (int j = 0; j < NSUB; ++j) {
+            P[b][k].x += g[j].x;
+            P[b][k].y += g[j].y;
+            g[j] = cmul(g[j], q[j]);
+          }
+        }
       }
     }
+  }
   }
   if (!valid) return;
 #pragma unroll
