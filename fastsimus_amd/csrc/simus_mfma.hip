@@ -923,6 +923,129 @@ __global__ __launch_bounds__(256) void k_msm_apply(Problem p, int E_pad, const M
   }
 }
 
+// The same product on the matrix cores, for at most 128 elements (one pair of element groups, K whole
+// in the first chunk's slot): per frequency X (E_pad x transmits) = K (E_pad x E_pad) A, a complex GEMM
+// in split bf16 as in k_msm (hi*hi + hi*lo + lo*hi, FP32 accumulation), with the complex parts in the
+// 32-deep contraction: Re X = [Re K | -Im K] . [Re A | Im A], Im X = [Im K | Re K] . [Re A | Im A].
+// Grid (frequency, phantom), 4 waves. The transmit weights of ATT 16-transmit tiles (at most) are
+// built once per pass into LDS as B fragments (lane l: transmit 16 tt + l % 16, elements
+// 16 c + 4 (l / 16) + i); every wave loads the K fragments of its ET / 4 row tiles straight from HBM
+// (lane l: row 16 a + l % 16, the 4 consecutive columns of its k-block) and accumulates its
+// ET / 4 x ATT output tiles.
+template <int ET, int ATT>
+__global__ __launch_bounds__(256) void k_msm_apply_mfma(Problem p, const MsmPhantoms phantoms, int nfpad,
+                                                        const cf* __restrict__ kpart, cf* __restrict__ out_all) {
+  constexpr int EP = ET * 16;
+  constexpr int RT = ET / 4;  // row tiles per wave
+  extern __shared__ __attribute__((aligned(16))) char lds_raw[];
+  u32x4* bfrag = (u32x4*)lds_raw;  // [tt][c][hl][lane]
+  const int k = blockIdx.x;
+  const int slot0 = phantoms.ph[blockIdx.y].x, nchunks = phantoms.ph[blockIdx.y].y;
+  cf* __restrict__ out = out_all + (size_t)blockIdx.y * p.NT * p.NF * p.E;
+  const cf* __restrict__ K = kpart + ((size_t)slot0 * nfpad + k) * EP * EP;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const double f = p.f0 + (double)k * p.df;
+  const u32x4 nim = {0u, 0u, 0x80008000u, 0x80008000u};
+  const double q = p.probe[k];
+  const cf scale = {(float)(p.pulse_re[k] * q * q * p.corr), (float)(p.pulse_im[k] * q * q * p.corr)};
+  for (int t0 = 0; t0 < p.NT; t0 += 16 * ATT) {
+    const int ntt = min(ATT, (p.NT - t0 + 15) / 16);
+    // B fragments of the transmit weights A(e', t) = w exp(i 2 pi f tau), zero past NT and E
+    for (int idx = tid; idx < ntt * ET * 64; idx += 256) {
+      const int l = idx & 63, c = (idx >> 6) % ET, tt = (idx >> 6) / ET;
+      const int t = t0 + 16 * tt + (l & 15);
+      float re[4], im[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int col = 16 * c + 4 * (l >> 4) + i;
+        re[i] = im[i] = 0.f;
+        if (t < p.NT && col < p.E) {
+          const double tau = p.delay[(size_t)t * p.E + col];
+          const double wgt = p.apod[(size_t)t * p.E + col];
+          const double nu = f * tau;
+          const float ph = (float)(nu - floor(nu));
+          float sn, cs;
+          sincospif(2.0f * ph, &sn, &cs);
+          re[i] = (float)wgt * cs;
+          im[i] = (float)wgt * sn;
+        }
+      }
+      unsigned h0, h1, h2, h3, l0, l1, l2, l3;
+      split_bf16_pair(re[0], re[1], h0, l0);
+      split_bf16_pair(re[2], re[3], h1, l1);
+      split_bf16_pair(im[0], im[1], h2, l2);
+      split_bf16_pair(im[2], im[3], h3, l3);
+      bfrag[((tt * ET + c) * 2 + 0) * 64 + l] = (u32x4){h0, h1, h2, h3};
+      bfrag[((tt * ET + c) * 2 + 1) * 64 + l] = (u32x4){l0, l1, l2, l3};
+    }
+    __syncthreads();
+    floatx4 acc[RT][ATT][2];
+#pragma unroll
+    for (int a = 0; a < RT; ++a)
+#pragma unroll
+      for (int tt = 0; tt < ATT; ++tt) acc[a][tt][0] = acc[a][tt][1] = (floatx4){0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < ET; ++c) {
+      u32x4 arh[RT], arl[RT], aih[RT], ail[RT];
+#pragma unroll
+      for (int a = 0; a < RT; ++a) {
+        const int row = 16 * (RT * w + a) + (lane & 15);
+        const int col = 16 * c + 4 * (lane >> 4);
+        float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
+        if (nchunks > 0) {
+          const float4* kr = (const float4*)(K + (size_t)row * EP + col);
+          v0 = kr[0];  // K[row][col], K[row][col + 1]
+          v1 = kr[1];  // K[row][col + 2], K[row][col + 3]
+        }
+        unsigned h0, h1, h2, h3, l0, l1, l2, l3;
+        split_bf16_pair(v0.x, v0.z, h0, l0);  // Re of columns col, col + 1
+        split_bf16_pair(v1.x, v1.z, h1, l1);  // Re of col + 2, col + 3
+        split_bf16_pair(v0.y, v0.w, h2, l2);  // Im
+        split_bf16_pair(v1.y, v1.w, h3, l3);
+        arh[a] = (u32x4){h0, h1, h2, h3} ^ nim;
+        arl[a] = (u32x4){l0, l1, l2, l3} ^ nim;
+        aih[a] = (u32x4){h2, h3, h0, h1};
+        ail[a] = (u32x4){l2, l3, l0, l1};
+      }
+#pragma unroll
+      for (int tt = 0; tt < ATT; ++tt) {
+        if (tt >= ntt) continue;  // uniform
+        const u32x4 bh = bfrag[((tt * ET + c) * 2 + 0) * 64 + lane];
+        const u32x4 bl = bfrag[((tt * ET + c) * 2 + 1) * 64 + lane];
+#pragma unroll
+        for (int a = 0; a < RT; ++a) {
+          floatx4 xr = acc[a][tt][0], xi = acc[a][tt][1];
+          xr = mma(arh[a], bh, xr);
+          xi = mma(aih[a], bh, xi);
+          xr = mma(arh[a], bl, xr);
+          xi = mma(aih[a], bl, xi);
+          xr = mma(arl[a], bh, xr);
+          xi = mma(ail[a], bh, xi);
+          acc[a][tt][0] = xr;
+          acc[a][tt][1] = xi;
+        }
+      }
+    }
+    // D layout: lane l holds rows (elements) 4 (l / 16) + i and column (transmit) l % 16 of its tile
+#pragma unroll
+    for (int tt = 0; tt < ATT; ++tt) {
+      if (tt >= ntt) continue;  // uniform
+      const int t = t0 + 16 * tt + (lane & 15);
+#pragma unroll
+      for (int a = 0; a < RT; ++a) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int e = 16 * (RT * w + a) + 4 * (lane >> 4) + i;
+          if (t < p.NT && e < p.E)
+            out[((size_t)t * p.NF + k) * p.E + e] = cmul(cf{acc[a][tt][0][i], acc[a][tt][1][i]}, scale);
+        }
+      }
+    }
+    __syncthreads();  // the next pass rewrites the B fragments
+  }
+}
+
 // ---------------------------------------------------------------- host side
 struct MsmPlan {
   int ET, FBK, nfb, nfpad, threads, grid, n_items;
@@ -1126,6 +1249,21 @@ size_t mfma_workspace(const Problem& p) {
   return mfma_workspace_batch(p, 1, off);
 }
 
+template <int ET, int ATT>
+static int launch_apply_mfma(const Problem& p, int P, const MsmPlan& m, const cf* kpart, cf* spect, hipStream_t st) {
+  auto kern = k_msm_apply_mfma<ET, ATT>;
+  constexpr int lds = ATT * ET * 2 * 64 * 16;
+  static bool lds_set[MAX_DEVICES] = {false};  // dynamic LDS above 64 KiB: set once per device
+  const int dev = current_device();
+  if (!lds_set[dev]) {
+    if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
+      return FS_ERR_RESOURCE;
+    lds_set[dev] = true;
+  }
+  hipLaunchKernelGGL(kern, dim3(p.NF, P), dim3(256), lds, st, p, m.phantoms, m.nfpad, kpart, spect);
+  return FS_OK;
+}
+
 template <int NSUB, int ET, int MODE, bool SHALLOW>
 static int launch_msm(const Problem& p, const MsmPlan& m, const float4* scat, const double* rt, const float* la,
                       const float2* qt, int* queue, cf* kpart, hipStream_t st) {
@@ -1215,7 +1353,11 @@ int mfma_spectrum_batch(const Problem& p, int P, const int64_t* off, cf* spect, 
   if (rc != FS_OK) return rc;
   hipLaunchKernelGGL(k_msm_fold, dim3(p.NF, P, m.npair), dim3(256), 0, st, m.ET * 16, m.nfpad, m.phantoms, kpart);
   const dim3 ag(p.NF, P, (p.E + 127) / 128);
-  if (p.NT <= 2)
+  if (m.groups <= 2 && p.NT > 16) {
+    // matrix cores: 5 transmit tiles per pass at 128 elements, 10 at 64 (80 KB of B fragments)
+    rc = m.ET == 8 ? launch_apply_mfma<8, 5>(p, P, m, kpart, spect, st) : launch_apply_mfma<4, 10>(p, P, m, kpart, spect, st);
+    if (rc != FS_OK) return rc;
+  } else if (p.NT <= 2)
     hipLaunchKernelGGL(k_msm_apply<1>, ag, dim3(256), 0, st, p, m.ET * 16, m.phantoms, m.nfpad, kpart, spect);
   else if (p.NT <= 16)
     hipLaunchKernelGGL(k_msm_apply<8>, ag, dim3(256), 0, st, p, m.ET * 16, m.phantoms, m.nfpad, kpart, spect);
