@@ -377,3 +377,30 @@ def test_hip_path_never_calls_the_oracle(cuda, monkeypatch):
     for strategy in ("hip", "hip_mfma"):
         rf, _ = _run(strategy, x, z, rc, D, params, medium, **kw)
         assert np.isfinite(rf).all() and np.abs(rf).max() > 0
+
+
+@pytest.mark.parametrize("n_elem,n_tx", [(48, 17), (64, 170), (100, 40), (128, 90)])
+def test_mfma_many_transmits_against_plain_and_fp64(n_elem, n_tx, cuda):
+    """Batched method with more than 16 transmits, where the multistatic matrix is applied to the transmits on
+    the matrix cores (64- and 128-element tiles; 170 and 90 transmits take two passes of transmit tiles):
+    against the plain FP32 method on every transmit and the FP64 oracle on two of them."""
+    p = fs.TransducerParams(freq_center=7.6e6, pitch=3.0e-4, width=2.7e-4, n_elements=n_elem, bandwidth=0.77)
+    m = fs.MediumParams(attenuation=0.5)
+    half = 0.5 * p.pitch * n_elem
+    x, z, rc = speckle(40 + n_elem, 300, xlim=(-half, half), zlim=(0.005, 0.03))
+    pos = _pos(p)
+    rng = np.random.default_rng(n_tx)
+    D = np.stack([fs.plane_wave(pos, float(a), speed_of_sound=1540.0) for a in np.deg2rad(np.linspace(-15, 15, n_tx))])
+    apod = rng.uniform(0.5, 1.0, D.shape)
+    sc = np.stack([x, z], -1)
+    rf_m = fs.simus(sc, rc, D, p, m, tx_apodization=apod, strategy="hip_mfma").rf.cpu().numpy()
+    rf_p = fs.simus(sc, rc, D, p, m, tx_apodization=apod, strategy="hip").rf.cpu().numpy()
+    e = rel_l2(rf_m, rf_p)
+    print(f"E={n_elem} n_tx={n_tx}: split-BF16 vs plain FP32 over all transmits rel-L2 {e:.2e}", flush=True)
+    assert e < 1e-4
+    plan = O.simus_plan(x, z, D, p, m)
+    for t in (0, n_tx - 1):
+        want, _, _ = O.simus_oracle(x, z, rc, D[t], p, m, plan, apod=apod[t])
+        err = rel_l2(rf_m[t], want)
+        print(f"  tx {t}: split-BF16 vs FP64 rel-L2 {err:.2e}", flush=True)
+        assert err < MFMA_TOL
