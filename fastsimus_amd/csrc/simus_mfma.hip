@@ -261,7 +261,8 @@ __global__ void k_scat_prep(Problem p, float4* __restrict__ out) {
 
 // ---------------------------------------------------------------- geometry table
 // Frequency-independent part of every (scatterer, element, sub-element) term,
-// evaluated once per call in FP64 (reference math: _pfield_math.py:71-247):
+// evaluated once per call (reference math: _pfield_math.py:71-247); the distance
+// and the phase reduction of the step in FP64, the values stored as FP32 in FP32:
 //   rt = r_c = max(r, c / (2 fc))                         (m, FP64)
 //   la = (log2(obliquity(theta) directivity(theta) |w| / sqrt(r_c)), float(r_c))
 //   qt = exp((-att + i 2 pi / c) r_c df)                  (the per-frequency step)
@@ -286,31 +287,36 @@ __global__ void k_geo_table(Problem p, const float4* __restrict__ scat, int EP, 
   const double w = fabs((double)sd.z);
 #pragma unroll
   for (int j = 0; j < NSUB; ++j) {
-    double rcl = p.min_dist, l = -INFINITY;
+    double rcl = p.min_dist;
+    float l = -INFINITY;
     if (e < p.E && w > 0.0) {
       const double dx = (double)sd.x - p.sub_x[e * NSUB + j];
       const double dz = (double)sd.y - p.sub_z[e * NSUB + j];
       const double r = sqrt(fma(dx, dx, dz * dz));
       rcl = fmax(r, p.min_dist);
-      const double sa = r > 0.0 ? dx / r : 1.0;
-      const double ca = r > 0.0 ? fabs(dz) / r : 0.0;
-      const double ce = p.ecos[e], se = p.esin[e];
-      const double st = sa * ce - ca * se;
-      const double ct = ca * ce + sa * se;
-      double obl = (p.baffle == FS_BAFFLE_SOFT) ? ct : (p.baffle == FS_BAFFLE_RIGID) ? 1.0 : ct / (ct + p.baffle_b);
-      if (!(ct > 0.0)) obl = 1e-16;
-      const double a = (double)p.kappa * st;
-      const double dir = fabs(a) < 1e-4 ? 1.0 - a * a / 6.0 : sin(a) / a;
-      const double amp = obl * dir * w / sqrt(rcl);
-      l = amp > 0.0 ? log2(amp) : -INFINITY;
+      const double ir = r > 0.0 ? 1.0 / r : 0.0;
+      // the angle terms and the amplitude are stored in FP32: evaluated in FP32 (library log2f, sinf)
+      const float sa = r > 0.0 ? (float)(dx * ir) : 1.f;
+      const float ca = (float)(fabs(dz) * ir);
+      const float ce = (float)p.ecos[e], se = (float)p.esin[e];
+      const float st = sa * ce - ca * se;
+      const float ct = ca * ce + sa * se;
+      float obl = (p.baffle == FS_BAFFLE_SOFT) ? ct : (p.baffle == FS_BAFFLE_RIGID) ? 1.f : ct / (ct + p.baffle_b);
+      if (!(ct > 0.f)) obl = 1e-16f;
+      const float a = p.kappa * st;
+      const float dir = fabsf(a) < 1e-2f ? fmaf(a * a, fmaf(a * a, 1.f / 120.f, -1.f / 6.f), 1.f) : sinf(a) / a;
+      const float amp = obl * dir * ((float)w / sqrtf((float)rcl));
+      l = amp > 0.f ? log2f(amp) : -INFINITY;
     }
-    double qs, qc;
-    sincospi(2.0 * (p.df * p.inv_c * rcl), &qs, &qc);
-    const double qm = exp(-(double)p.att_coef * p.df * rcl);
+    // per-frequency step: phase reduced to [-1/2, 1/2] revolution in FP64, then FP32 sincospif / exp2f
+    const double rev = p.df * p.inv_c * rcl;
+    float qs, qc;
+    sincospif(2.f * (float)(rev - rint(rev)), &qs, &qc);
+    const float qm = exp2f((float)(-(double)p.att_coef * (double)kLog2e * p.df * rcl));
     rt[idx * NSUB + j] = rcl;
-    la[2 * (idx * NSUB + j)] = (float)l;
+    la[2 * (idx * NSUB + j)] = l;
     la[2 * (idx * NSUB + j) + 1] = (float)rcl;
-    qt[idx * NSUB + j] = make_float2((float)(qm * qc), (float)(qm * qs));
+    qt[idx * NSUB + j] = make_float2(qm * qc, qm * qs);
   }
 }
 
