@@ -465,13 +465,13 @@ __global__ void k_adjm_conj(Problem p, int nfp, const cf* __restrict__ gspect, c
 
 // Split-BF16 fragments of R for transmit group q, per frequency and element slice w:
 //   R1[k][w][n][hl][lane]: B operand of Y = G R, lane (m = 16 n + l % 16; elements 16 w + 4 (l / 16) + c),
-//     as [Re of the 4 | Im of the 4];
+//     as [Re of the 4 | Im of the 4], times c_k PP_k (so that the product is c PP Y);
 //   R2[k][w][x][hl][lane]: A operand of W^T = R L^T, lane (element 16 w + l % 16; m = 16 x + 4 (l / 16) + c),
 //     interleaved: word c = (Re, Im) of entry c (the order of the contraction index inside a fragment is
 //     free as long as both operands share it; the L^T operands are written in the same order);
 // zero for t >= NT, e >= E and k >= NF (k < nfp).
 __global__ void k_adjm_rtab(Problem p, int nfp, int q, int nw, const cf* __restrict__ A, const cf* __restrict__ Gc,
-                            am_u32x4* __restrict__ R1, am_u32x4* __restrict__ R2) {
+                            const cf* __restrict__ ppc, am_u32x4* __restrict__ R1, am_u32x4* __restrict__ R2) {
   const long total = (long)nfp * nw * 2 * 64;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
     const int lane = (int)(i & 63);
@@ -486,11 +486,14 @@ __global__ void k_adjm_rtab(Problem p, int nfp, int q, int nw, const cf* __restr
     // R1: one transmit, four elements
     {
       const int t = AM_TG * q + col;
+      const cf pp = ppc[2 * k];
+      const float ck = ppc[2 * k + 1].x;
+      const cf cpp = {ck * pp.x, ck * pp.y};
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const int e = 16 * w + 4 * g + c;
         cf v = {0.f, 0.f};
-        if (t < p.NT && e < p.E && k < p.NF) v = src[((size_t)t * p.E + e) * nfp + k];
+        if (t < p.NT && e < p.E && k < p.NF) v = cmul(cpp, src[((size_t)t * p.E + e) * nfp + k]);
         re[c] = v.x;
         im[c] = v.y;
       }
@@ -522,23 +525,23 @@ __global__ void k_adjm_rtab(Problem p, int nfp, int q, int nw, const cf* __restr
   }
 }
 
-// Operands of Y = G R for one frequency: G split as [Re | -Im] (real part) and [Im | Re]
-// (imaginary part), hi and lo, and G itself for the rc gradient.
+// Operands of Y = G R for one frequency: G = sum_j amp_j E_j (the amplitudes ride in the recurrence: Ea_j =
+// amp_j E_j) split as [Re | -Im] (real part) and [Im | Re] (imaginary part), hi and lo.
 struct AmOps {
   am_u32x4 a1h, a1l, a2h, a2l;
   float gr[4], gi[4];
 };
 
 template <int NSUB>
-__device__ __forceinline__ void am_gen(const float (&amp)[4][NSUB], const cf (&E)[4][NSUB], AmOps& o) {
+__device__ __forceinline__ void am_gen(const cf (&Ea)[4][NSUB], AmOps& o) {
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    o.gr[c] = amp[c][0] * E[c][0].x;
-    o.gi[c] = amp[c][0] * E[c][0].y;
+    o.gr[c] = Ea[c][0].x;
+    o.gi[c] = Ea[c][0].y;
 #pragma unroll
     for (int j = 1; j < NSUB; ++j) {
-      o.gr[c] = fmaf(amp[c][j], E[c][j].x, o.gr[c]);
-      o.gi[c] = fmaf(amp[c][j], E[c][j].y, o.gi[c]);
+      o.gr[c] += Ea[c][j].x;
+      o.gi[c] += Ea[c][j].y;
     }
   }
   const am_u32x4 nim = {0u, 0u, 0x80008000u, 0x80008000u};
@@ -574,14 +577,15 @@ __device__ __forceinline__ void am_mfma1(const AmOps& o, const am_u32x4 (&b1)[2]
   }
 }
 
-// Y of every slice summed in a fixed order; L(s, m ^ 16) = c PP Y(s, m) written as the two
+// Y of every slice summed in a fixed order (R1 carries c PP, so the sum is c PP Y); L(s, m ^ 16) = c PP Y(s, m)
+// written as the two
 // split-BF16 B operands of W^T = R L^T in the interleaved order of R2: word c of a lane's fragment is
 // (Re L, -Im L) of its entry c (real part) or (Im L, Re L) (imaginary part), so each entry is one
 // 32-bit word per operand: the sign flip of the upper half or the swap of the halves of its split pair.
 // A lane writes 4 words (hi and lo of both parts); the 64 lanes of a wave write 64 consecutive words.
 // LDS: ypr / ypi [w][m][s] (rows of AM_YROW floats); lfrag [x][part][hl][lane][4 words].
 template <int NW>
-__device__ __forceinline__ void am_reduce(const float* ypr, const float* ypi, unsigned short* lfrag, cf cpp) {
+__device__ __forceinline__ void am_reduce(const float* ypr, const float* ypi, unsigned short* lfrag) {
 #pragma unroll
   for (int r = 0; r < 512 / (64 * NW); ++r) {
     const int ent = threadIdx.x + r * 64 * NW;
@@ -598,7 +602,7 @@ __device__ __forceinline__ void am_reduce(const float* ypr, const float* ypi, un
       yx += tr[v];
       yy += ti[v];
     }
-    const cf Lv = cmul(cpp, cf{yx, yy});
+    const cf Lv = {yx, yy};
     const int mt = m ^ 16;
     const int xx = mt >> 4, mm = mt & 15;
     const int ln = sl + 16 * (mm >> 2), cc = mm & 3;
@@ -627,7 +631,7 @@ __device__ __forceinline__ void am_reduce(const float* ypr, const float* ypi, un
 // register sets (the step sequence is unrolled by three), so nothing is copied.
 template <int NSUB, int NW>
 __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, const am_u32x4* __restrict__ R1,
-                                                  const am_u32x4* __restrict__ R2, const cf* __restrict__ ppc,
+                                                  const am_u32x4* __restrict__ R2,
                                                   float* __restrict__ prc, float* __restrict__ px,
                                                   float* __restrict__ pz, float* __restrict__ pc) {
   static_assert(AM_FC % 3 == 2, "the steady steps come in groups of three");
@@ -649,8 +653,10 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
   const float zs = valid ? p.z[s] : 1e-2f;
   const bool in_field = valid && !out_of_field(p, xs, zs);
 
-  // E_j(f) = exp(f r_c (-alpha' + i 2 pi / c)) by FP32 recurrence from an FP64-phase seed at the start
-  // of every chunk; the amplitude and the per-step factor hold for all chunks
+  // amp_j E_j(f) = amp_j exp(f r_c (-alpha' + i 2 pi / c)) by FP32 recurrence from an FP64-phase seed at
+  // the start of every chunk; the amplitude and the per-step factor hold for all chunks. An in-field term of
+  // zero amplitude (a zero of the directivity) runs with amplitude 1e-20 instead, so that its S0 and S1 can
+  // be recovered by dividing by the amplitude (its G differs from zero by 1e-20 |E|).
   cf EA[4][NSUB], EB[4][NSUB], EC[4][NSUB], Q[4][NSUB];
   float amp[4][NSUB];
 #pragma unroll
@@ -659,22 +665,24 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
       const GeoF geo = geometry_fast(p, xs, zs, e < p.E ? e : p.E - 1, j);
-      amp[c][j] = (in_field && e < p.E) ? geo.amp : 0.f;
+      amp[c][j] = (in_field && e < p.E) ? (geo.amp != 0.f ? geo.amp : 1e-20f) : 0.f;
       Q[c][j] = green_step_fast(geo, p.df);
       EA[c][j] = EB[c][j] = EC[c][j] = {0.f, 0.f};
     }
   }
-  cf S0[4][NSUB], S1[4][NSUB];
+  // S0_j = sum_f amp_j E_j W over the item's frequencies f_0 .. f_K (in order), and V_j = sum_k of the partial
+  // sums of S0_j before step k, so that sum_f f amp_j E_j W = f_K S0_j - df V_j (summation by parts)
+  cf S0[4][NSUB], V[4][NSUB];
 #pragma unroll
   for (int c = 0; c < 4; ++c)
 #pragma unroll
-    for (int j = 0; j < NSUB; ++j) S0[c][j] = S1[c][j] = {0.f, 0.f};
+    for (int j = 0; j < NSUB; ++j) S0[c][j] = V[c][j] = {0.f, 0.f};
   am_u32x4 b1[2][2], a2[2][2];
 
   // stage A(i): operands and Y of f_i from E(f_i); stored to Y buffer i % 2
   auto stage_a = [&](int i, const cf (&Ei)[4][NSUB]) {
     AmOps o;
-    am_gen<NSUB>(amp, Ei, o);
+    am_gen<NSUB>(Ei, o);
     am_floatx4 yr[2], yi[2];
     am_mfma1(o, b1, yr, yi);
     const int kn = min(k0 + i + 1, klast);
@@ -691,11 +699,7 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
     }
   };
   // stage B(i): L of f_i from Y buffer i % 2 into L buffer i % 2
-  auto stage_b = [&](int i) {
-    const cf pp = ppc[2 * (k0 + i)];
-    const float ck = ppc[2 * (k0 + i) + 1].x;
-    am_reduce<NW>(ypr + (i & 1) * YB, ypi + (i & 1) * YB, lfrag + (i & 1) * LB, cf{ck * pp.x, ck * pp.y});
-  };
+  auto stage_b = [&](int i) { am_reduce<NW>(ypr + (i & 1) * YB, ypi + (i & 1) * YB, lfrag + (i & 1) * LB); };
   // stage C(i): W^T of f_i (A = R as stored, B = the two variants of L^T; result lane = the lane's
   // own terms) and the terms of f_i
   auto stage_c = [&](int i, const cf (&Ei)[4][NSUB]) {
@@ -719,17 +723,16 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
     for (int x = 0; x < 2; ++x)
 #pragma unroll
       for (int hl = 0; hl < 2; ++hl) a2[x][hl] = R2[am_ridx(kn, nw, w, x, hl, lane)];
-    const float f = (float)(p.f0 + (double)(k0 + i) * p.df);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const cf Wc = {wr[c], wi[c]};
 #pragma unroll
       for (int j = 0; j < NSUB; ++j) {
-        const cf ew = cmul(Ei[c][j], Wc);
-        S0[c][j].x += ew.x;
-        S0[c][j].y += ew.y;
-        S1[c][j].x = fmaf(f, ew.x, S1[c][j].x);
-        S1[c][j].y = fmaf(f, ew.y, S1[c][j].y);
+        const cf ea = Ei[c][j];
+        V[c][j].x += S0[c][j].x;
+        V[c][j].y += S0[c][j].y;
+        S0[c][j].x = fmaf(ea.x, Wc.x, fmaf(-ea.y, Wc.y, S0[c][j].x));
+        S0[c][j].y = fmaf(ea.x, Wc.y, fmaf(ea.y, Wc.x, S0[c][j].y));
       }
     }
   };
@@ -759,7 +762,7 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
     const int e = 16 * w + 4 * g + c;
 #pragma unroll
     for (int j = 0; j < NSUB; ++j)
-      EC[c][j] = green_fast(geometry_fast(p, xs, zs, e < p.E ? e : p.E - 1, j), fstart, 1.0f);
+      EC[c][j] = green_fast(geometry_fast(p, xs, zs, e < p.E ? e : p.E - 1, j), fstart, amp[c][j]);
   }
 #pragma unroll
   for (int x = 0; x < 2; ++x)
@@ -792,16 +795,20 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
   }  // chunks
 
   // derivative coefficients once per item; sums over the lane's terms, its 4 lane groups and the slices.
-  // The amplitudes do not depend on frequency, so sum_f Re(G W) = sum_j amp_j Re S0_j: the rc gradient
-  // comes from the same sums (amp = 0 for padded elements and out-of-field scatterers).
+  // The amplitudes do not depend on frequency, so sum_f Re(G W) = sum_j Re S0_j: the rc gradient comes from
+  // the same sums (S0 = 0 for padded elements and out-of-field scatterers). The position and speed-of-sound
+  // terms take S0_j / amp_j and S1_j / amp_j (amp_j != 0 for every in-field term of a real element).
   float racc = 0.f;
 #pragma unroll
   for (int c = 0; c < 4; ++c)
 #pragma unroll
-    for (int j = 0; j < NSUB; ++j) racc = fmaf(amp[c][j], S0[c][j].x, racc);
+    for (int j = 0; j < NSUB; ++j) racc += S0[c][j].x;
   float gx = 0.f, gz = 0.f, gcs = 0.f;
   if (in_field) {
     const float fc_hz = (float)(p.min_dist > 0 ? 0.5 / (p.min_dist * p.inv_c) : 1.0);
+    const int kend = min((int)(blockIdx.y + 1) * AM_CPI, nch) * AM_FC - 1;  // the item's last frequency
+    const float fend = (float)(p.f0 + (double)kend * p.df);
+    const float dff = (float)p.df;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const int e = 16 * w + 4 * g + c;
@@ -811,9 +818,12 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
         GeoF geo;
         Dcoef dc;
         coeffs_fast(p, xs, zs, e, j, fc_hz, geo, dc);
-        gx += dc.ax * S0[c][j].x + (dc.bx.x * S1[c][j].x - dc.bx.y * S1[c][j].y);
-        gz += dc.az * S0[c][j].x + (dc.bz.x * S1[c][j].x - dc.bz.y * S1[c][j].y);
-        gcs += dc.ac * S0[c][j].x + (dc.bc.x * S1[c][j].x - dc.bc.y * S1[c][j].y);
+        const float ia = 1.0f / amp[c][j];
+        const float s0 = S0[c][j].x * ia;
+        const cf s1 = {(fend * S0[c][j].x - dff * V[c][j].x) * ia, (fend * S0[c][j].y - dff * V[c][j].y) * ia};
+        gx += dc.ax * s0 + (dc.bx.x * s1.x - dc.bx.y * s1.y);
+        gz += dc.az * s0 + (dc.bz.x * s1.x - dc.bz.y * s1.y);
+        gcs += dc.ac * s0 + (dc.bc.x * s1.x - dc.bc.y * s1.y);
       }
     }
   }
@@ -984,16 +994,16 @@ static int adjoint_matrix(const Problem& p, const cf* dspect, float* d_rc, float
   const dim3 grid((p.S + 15) / 16, L.nitem);
   for (int q = 0; q < L.ngrp; ++q) {
     hipLaunchKernelGGL(k_adjm_rtab, dim3(rblocks), dim3(256), 0, st, p, L.nfp, q, L.nw, (const cf*)A,
-                       (const cf*)Gc, R1, R2);
+                       (const cf*)Gc, (const cf*)ppc, R1, R2);
     const dim3 block(64 * L.nw);
     if (p.NS == 1 && L.nw == 4)
-      hipLaunchKernelGGL((k_adjm<1, 4>), grid, block, 0, st, p, q, L.nch, R1, R2, ppc, prc, px, pz, pc);
+      hipLaunchKernelGGL((k_adjm<1, 4>), grid, block, 0, st, p, q, L.nch, R1, R2, prc, px, pz, pc);
     else if (p.NS == 1)
-      hipLaunchKernelGGL((k_adjm<1, 8>), grid, block, 0, st, p, q, L.nch, R1, R2, ppc, prc, px, pz, pc);
+      hipLaunchKernelGGL((k_adjm<1, 8>), grid, block, 0, st, p, q, L.nch, R1, R2, prc, px, pz, pc);
     else if (L.nw == 4)
-      hipLaunchKernelGGL((k_adjm<2, 4>), grid, block, 0, st, p, q, L.nch, R1, R2, ppc, prc, px, pz, pc);
+      hipLaunchKernelGGL((k_adjm<2, 4>), grid, block, 0, st, p, q, L.nch, R1, R2, prc, px, pz, pc);
     else
-      hipLaunchKernelGGL((k_adjm<2, 8>), grid, block, 0, st, p, q, L.nch, R1, R2, ppc, prc, px, pz, pc);
+      hipLaunchKernelGGL((k_adjm<2, 8>), grid, block, 0, st, p, q, L.nch, R1, R2, prc, px, pz, pc);
   }
   const int nb = L.ngrp * L.nitem;
   const dim3 g1((p.S + 255) / 256);
