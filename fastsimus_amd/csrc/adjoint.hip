@@ -679,17 +679,18 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
     for (int j = 0; j < NSUB; ++j) S0[c][j] = V[c][j] = {0.f, 0.f};
   am_u32x4 b1[2][2], a2[2][2];
 
-  // stage A(i): operands and Y of f_i from E(f_i); stored to Y buffer i % 2
-  auto stage_a = [&](int i, const cf (&Ei)[4][NSUB]) {
+  // stage A(i): operands and Y of f_i from E(f_i) (matrix products), then stored to Y buffer i % 2
+  auto stage_a_mma = [&](int i, const cf (&Ei)[4][NSUB], am_floatx4 (&yr)[2], am_floatx4 (&yi)[2]) {
     AmOps o;
     am_gen<NSUB>(Ei, o);
-    am_floatx4 yr[2], yi[2];
     am_mfma1(o, b1, yr, yi);
     const int kn = min(k0 + i + 1, klast);
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
       for (int hl = 0; hl < 2; ++hl) b1[x][hl] = R1[am_ridx(kn, nw, w, x, hl, lane)];
+  };
+  auto stage_a_store = [&](int i, const am_floatx4 (&yr)[2], const am_floatx4 (&yi)[2]) {
     float* yb = ypr + (i & 1) * YB;
     float* ib = ypi + (i & 1) * YB;
 #pragma unroll
@@ -698,13 +699,19 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
       *(am_floatx4*)(ib + (w * 32 + 16 * x + col) * AM_YROW + 4 * (g ^ (col >> 2))) = yi[x];
     }
   };
+  auto stage_a = [&](int i, const cf (&Ei)[4][NSUB]) {
+    am_floatx4 yr[2], yi[2];
+    stage_a_mma(i, Ei, yr, yi);
+    stage_a_store(i, yr, yi);
+  };
   // stage B(i): L of f_i from Y buffer i % 2 into L buffer i % 2
   auto stage_b = [&](int i) { am_reduce<NW>(ypr + (i & 1) * YB, ypi + (i & 1) * YB, lfrag + (i & 1) * LB); };
-  // stage C(i): W^T of f_i (A = R as stored, B = the two variants of L^T; result lane = the lane's
-  // own terms) and the terms of f_i
-  auto stage_c = [&](int i, const cf (&Ei)[4][NSUB]) {
+  // stage C(i): W^T of f_i (A = R as stored, B = the two variants of L^T; result lane = the lane's own
+  // terms), then the terms of f_i
+  auto stage_c_mma = [&](int i, am_floatx4& wr, am_floatx4& wi) {
     const am_u32x4* lf4 = (const am_u32x4*)(lfrag + (i & 1) * LB);
-    am_floatx4 wr = {0.f, 0.f, 0.f, 0.f}, wi = {0.f, 0.f, 0.f, 0.f};
+    wr = (am_floatx4){0.f, 0.f, 0.f, 0.f};
+    wi = (am_floatx4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
       const am_u32x4 brh = lf4[(x * 4 + 0) * 64 + lane];
@@ -723,6 +730,8 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
     for (int x = 0; x < 2; ++x)
 #pragma unroll
       for (int hl = 0; hl < 2; ++hl) a2[x][hl] = R2[am_ridx(kn, nw, w, x, hl, lane)];
+  };
+  auto stage_c_acc = [&](const cf (&Ei)[4][NSUB], const am_floatx4& wr, const am_floatx4& wi) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const cf Wc = {wr[c], wi[c]};
@@ -736,17 +745,27 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
       }
     }
   };
+  auto stage_c = [&](int i, const cf (&Ei)[4][NSUB]) {
+    am_floatx4 wr, wi;
+    stage_c_mma(i, wr, wi);
+    stage_c_acc(Ei, wr, wi);
+  };
   auto advance = [&](const cf (&Ei)[4][NSUB], cf (&Eo)[4][NSUB]) {
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
       for (int j = 0; j < NSUB; ++j) Eo[c][j] = cmul(Ei[c][j], Q[c][j]);
   };
-  // steady step i (2 <= i < AM_FC): Ea = E(f_{i-2}) (then E(f_{i+1})), Ec = E(f_i)
+  // steady step i (2 <= i < AM_FC): Ea = E(f_{i-2}) (then E(f_{i+1})), Ec = E(f_i). The matrix products of
+  // C(i - 2) and A(i) are issued first; the vector work of B(i - 1) and of the terms of f_{i-2} runs while
+  // they complete, and Y(f_i) is stored last.
   auto steady = [&](int i, cf (&Ea)[4][NSUB], cf (&Ec)[4][NSUB]) {
-    stage_a(i, Ec);
+    am_floatx4 wr, wi, yr[2], yi[2];
+    stage_c_mma(i - 2, wr, wi);
+    stage_a_mma(i, Ec, yr, yi);
     stage_b(i - 1);
-    stage_c(i - 2, Ea);
+    stage_c_acc(Ea, wr, wi);
+    stage_a_store(i, yr, yi);
     advance(Ec, Ea);
     __syncthreads();
   };
