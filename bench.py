@@ -292,23 +292,56 @@ def bench_adjoint(args) -> int:
         loss.backward()
         return loss
 
+    from fastsimus_amd import _lib
+
+    lib = _lib.load()
+
     def timed(adjoint, steps, warmup):
+        """Seconds per step, and ms per step and launches of the backward's dominant kernels (k_adjm per
+        transmit group, or k_adj_tx + k_adj_pos per transmit) by HIP events on the stream they run on."""
         for _ in range(warmup):
             step(adjoint)
         torch.cuda.synchronize()
+        lib.fs_set_kernel_timing(2)  # FS_TIMING_ADJOINT
+        _lib.kernel_time_ms()
         t0 = time.perf_counter()
         for _ in range(steps):
             step(adjoint)
         torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / steps
+        dt = (time.perf_counter() - t0) / steps
+        k_ms, launches = _lib.kernel_time_ms()
+        lib.fs_set_kernel_timing(0)
+        return dt, k_ms / steps, launches
 
-    t_step = timed(args.adjoint, args.steps, args.warmup)
+    t_step, k_ms, k_launches = timed(args.adjoint, args.steps, args.warmup)
     # the other backward kernels on the same step, for comparison (FP32 vector kernels regenerate the
     # Green's functions per transmit; the matrix-core form once per group of 16 transmits)
     other = "vector" if args.adjoint != "vector" else "matrix"
-    t_other = timed(other, 2, 1)
+    t_other, k_other, _ = timed(other, 2, 1)
     nf = int(plan.selected_freqs.shape[0])
-    value = D.shape[0] * x.size * params.n_elements * nf / t_step / 1e9
+    n_tx, S, E = int(D.shape[0]), int(x.size), params.n_elements
+    value = n_tx * S * E * nf / t_step / 1e9
+    roofline = None
+    if args.adjoint != "vector":
+        # k_adjm: per workgroup (16 scatterers, 8 element slices of 16) and padded frequency, 24 MFMA
+        # 16x16x32 per slice (Y = G R and W^T = R L^T, three split-bf16 products each for the real and the
+        # imaginary part); one launch per group of 16 transmits, frequencies padded to 32-frequency chunks
+        nch = -(-nf // 32)
+        slices = 4 if E <= 64 else 8
+        groups = -(-n_tx // 16)
+        issued = groups * (-(-S // 16)) * nch * 32 * slices * 24 * 16 * 16 * 32 * 2
+        # algorithmic flops of the transmit contractions: P_t and B_t (two complex MACs per (t, s, e, f)) and
+        # W = sum_t (two complex MACs per (t, s, e, f))
+        useful = 32 * n_tx * S * E * nf
+        roofline = {"bound": "mfma", "achieved": useful / (k_ms * 1e-3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
+                    "frac": useful / (k_ms * 1e-3) / 2.5e15, "traffic": None,
+                    "kernel": "k_adjm (split-BF16 v_mfma_f32_16x16x32_bf16 + FP32 VALU terms)",
+                    "kernel_ms": k_ms, "launches": k_launches, "flops_per_launch": useful / groups,
+                    "flops_definition": "32 n_tx S E n_f useful flops (P_t, B_t and the transmit sum of W: two "
+                                        "complex MACs each per (t, s, e, f)) / k_adjm HIP-event duration",
+                    "mfma_issued_flops_per_launch": issued / groups,
+                    "mfma_issue_tflops": issued / (k_ms * 1e-3) / 1e12,
+                    "mfma_issue_frac": issued / (k_ms * 1e-3) / 2.5e15}
     out = {
         "metric": "G(scatterer·element·freq)/s, forward + adjoint (dL/d rc, x, z, c)", "value": value, "unit": UNIT,
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_step * 1e3,
@@ -319,8 +352,8 @@ def bench_adjoint(args) -> int:
                    "n_scat": int(x.size), "n_elem": params.n_elements, "n_tx": int(D.shape[0]), "n_freq": nf,
                    "n_sub": plan.n_sub,
                    "method": "hip_mfma forward (multistatic matrix) + hand-written adjoint", "adjoint": args.adjoint},
-        f"{other}_adjoint_ms_per_step": t_other * 1e3,
-        "roofline": None, "cpu_baseline": None,
+        f"{other}_adjoint_ms_per_step": t_other * 1e3, f"{other}_adjoint_kernel_ms": k_other,
+        "roofline": roofline, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -923,8 +923,10 @@ static void adj_launch(const Problem& p, const AdjLayout& L, char* ws, int t, fl
   float* dx = (float*)(ws + L.odx);
   float* dz = (float*)(ws + L.odz);
   float* dcp = (float*)(ws + L.odc) + (size_t)t * L.nfb * p.S;
+  const int slot = timing_start(st, FS_TIMING_ADJOINT);
   hipLaunchKernelGGL((k_adj_tx<NSUB, FD, GR>), grid, dim3(256), 0, st, p, t, L.nfp, A, Gc, ppc, Pb, Bb, drc);
   hipLaunchKernelGGL((k_adj_pos<NSUB, FD, GR>), grid, dim3(256), 0, st, p, t, L.nfp, TA, TG, Pb, Bb, dx, dz, dcp);
+  timing_stop(slot, st);
   const dim3 g1((p.S + 255) / 256);
   hipLaunchKernelGGL(k_sum_parts, g1, dim3(256), 0, st, drc, L.nfb, p.S, t > 0 ? 1 : 0, d_rc);
   hipLaunchKernelGGL(k_sum_parts, g1, dim3(256), 0, st, dx, L.nfb, p.S, t > 0 ? 1 : 0, d_x);
@@ -1015,6 +1017,7 @@ static int adjoint_matrix(const Problem& p, const cf* dspect, float* d_rc, float
     hipLaunchKernelGGL(k_adjm_rtab, dim3(rblocks), dim3(256), 0, st, p, L.nfp, q, L.nw, (const cf*)A,
                        (const cf*)Gc, (const cf*)ppc, R1, R2);
     const dim3 block(64 * L.nw);
+    const int slot = timing_start(st, FS_TIMING_ADJOINT);
     if (p.NS == 1 && L.nw == 4)
       hipLaunchKernelGGL((k_adjm<1, 4>), grid, block, 0, st, p, q, L.nch, R1, R2, prc, px, pz, pc);
     else if (p.NS == 1)
@@ -1023,6 +1026,7 @@ static int adjoint_matrix(const Problem& p, const cf* dspect, float* d_rc, float
       hipLaunchKernelGGL((k_adjm<2, 4>), grid, block, 0, st, p, q, L.nch, R1, R2, prc, px, pz, pc);
     else
       hipLaunchKernelGGL((k_adjm<2, 8>), grid, block, 0, st, p, q, L.nch, R1, R2, prc, px, pz, pc);
+    timing_stop(slot, st);
   }
   const int nb = L.ngrp * L.nitem;
   const dim3 g1((p.S + 255) / 256);

@@ -156,7 +156,7 @@ __device__ __forceinline__ void split_bf16_pair(float a, float b, unsigned& hi, 
   lo = __builtin_bit_cast(unsigned, __builtin_convertvector((fs_floatx2){a - ah, b - bh}, fs_bf16x2));
 }
 
-int timing_start(hipStream_t st);
+int timing_start(hipStream_t st, int cls = FS_TIMING_SPECTRUM);
 void timing_stop(int slot, hipStream_t st);
 
 }  // namespace fs

@@ -6,13 +6,13 @@
 namespace fs {
 
 constexpr int kSlots = 512;
-static bool g_on = false;
+static int g_mask = 0;  // FS_TIMING_* classes being timed
 static bool g_init = false;
 static hipEvent_t g_start[kSlots], g_stop[kSlots];
 static int g_used = 0;
 
-int timing_start(hipStream_t st) {
-  if (!g_on || g_used >= kSlots) return -1;
+int timing_start(hipStream_t st, int cls) {
+  if (!(g_mask & cls) || g_used >= kSlots) return -1;
   const int s = g_used++;
   hipEventRecord(g_start[s], st);
   return s;
@@ -34,7 +34,7 @@ int fs_set_kernel_timing(int enable) {
     }
     g_init = true;
   }
-  g_on = enable != 0;
+  g_mask = enable;
   g_used = 0;
   return FS_OK;
 }

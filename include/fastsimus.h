@@ -238,11 +238,14 @@ int fs_das(const float* iq, int n_firings, int n_samples, int n_channels, int ch
            double fs_hz, double fc_hz, double speed_of_sound, double t0, double f_number, int compound, float* image,
            void* stream);
 
-/* Optional timing of the dominant kernel launches (k_msm for FS_METHOD_MFMA_BF16X3, k_tx + k_rx for
- * FS_METHOD_FP32): while enabled, each such launch is bracketed by HIP events on its stream.
+/* Optional timing of the dominant kernel launches. enable is a mask of classes:
+ * FS_TIMING_SPECTRUM (1): k_msm for FS_METHOD_MFMA_BF16X3, k_tx + k_rx for FS_METHOD_FP32;
+ * FS_TIMING_ADJOINT (2): k_adjm per transmit group, or k_adj_tx + k_adj_pos per transmit;
+ * 0 turns timing off. While enabled, each such launch is bracketed by HIP events on its stream.
  * fs_kernel_time_ms waits for the recorded launches, returns their summed duration in milliseconds,
  * stores the launch count in *launches (if not NULL) and clears the record. Process-wide and not
  * thread-safe (meant for benchmarks). No counterpart in the reference. */
+enum fs_timing_class { FS_TIMING_SPECTRUM = 1, FS_TIMING_ADJOINT = 2 };
 int fs_set_kernel_timing(int enable);
 double fs_kernel_time_ms(int* launches);
 
