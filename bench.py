@@ -818,8 +818,8 @@ def main() -> int:
         "ms_per_step": t_step * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16", "precision": "split-BF16 (hi+lo, 3 products) MFMA, FP32 accumulate; FP64 r and phase reseed",
         "data": "synthetic",
-        "config": {"workload": "batched plane-wave compounding: 128-element 7.6 MHz linear array, 75 plane waves "
-                               "in [-16,16] deg, 200000 speckle scatterers, 0.5 dB/cm/MHz",
+        "config": {"workload": f"batched plane-wave compounding: 128-element 7.6 MHz linear array, {T} plane waves "
+                               f"in [-16,16] deg, {S} speckle scatterers, 0.5 dB/cm/MHz",
                    "n_scat": S, "n_elem": E, "n_tx": T, "n_freq": nf, "n_sub": plan.n_sub, "n_fft": plan.n_fft,
                    "method": "hip_mfma", "tiles": "autotuned" if tuning else "library defaults",
                    "launch": "hipGraph replay" if graph is not None else "eager",

@@ -656,15 +656,20 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
   // amp_j E_j(f) = amp_j exp(f r_c (-alpha' + i 2 pi / c)) by FP32 recurrence from an FP64-phase seed at
   // the start of every chunk; the amplitude and the per-step factor hold for all chunks. An in-field term of
   // zero amplitude (a zero of the directivity) runs with amplitude 1e-20 instead, so that its S0 and S1 can
-  // be recovered by dividing by the amplitude (its G differs from zero by 1e-20 |E|).
+  // be recovered by dividing by the amplitude (its G differs from zero by 1e-20 |E|). The delay r_c / c and the
+  // attenuation of each term are kept for the chunks' seeds, so that the geometry is evaluated once per item.
   cf EA[4][NSUB], EB[4][NSUB], EC[4][NSUB], Q[4][NSUB];
   float amp[4][NSUB];
+  double tt[4][NSUB];
+  float ta[4][NSUB];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const int e = 16 * w + 4 * g + c;
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
       const GeoF geo = geometry_fast(p, xs, zs, e < p.E ? e : p.E - 1, j);
+      tt[c][j] = geo.t;
+      ta[c][j] = geo.att;
       amp[c][j] = (in_field && e < p.E) ? (geo.amp != 0.f ? geo.amp : 1e-20f) : 0.f;
       Q[c][j] = green_step_fast(geo, p.df);
       EA[c][j] = EB[c][j] = EC[c][j] = {0.f, 0.f};
@@ -777,12 +782,9 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
   k0 = chunk * AM_FC;
   const double fstart = p.f0 + (double)k0 * p.df;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int e = 16 * w + 4 * g + c;
+  for (int c = 0; c < 4; ++c)
 #pragma unroll
-    for (int j = 0; j < NSUB; ++j)
-      EC[c][j] = green_fast(geometry_fast(p, xs, zs, e < p.E ? e : p.E - 1, j), fstart, amp[c][j]);
-  }
+    for (int j = 0; j < NSUB; ++j) EC[c][j] = green_fast(GeoF{tt[c][j], 0.f, ta[c][j], 0.f}, fstart, amp[c][j]);
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
