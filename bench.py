@@ -43,8 +43,8 @@ UNIT = "G(scatterer·element·freq)/s"
 PEAK_BF16 = 2.5e15    # MI355X dense BF16 MFMA, FLOP/s
 PEAK_FP32 = 157.3e12  # MI355X FP32 vector, FLOP/s
 # VALU instructions the plain kernels (k_tx + k_rx) issue per evaluation at n_sub = 2 on the
-# benchmarked sizes (SQ_INSTS_VALU x 64 / evaluations: k_tx 15.1, k_rx 20.7)
-VALU_PER_EVAL_NSUB2 = 35.8
+# benchmarked sizes (SQ_INSTS_VALU x 64 / evaluations: k_tx 15.1, k_rx 16.8 with the geometry table)
+VALU_PER_EVAL_NSUB2 = 31.9
 
 
 def workload(seed: int = 2026, n_scat: int = 200_000, n_tx: int = 75):
@@ -788,9 +788,9 @@ def main() -> int:
              "roofline": {"bound": "valu", "achieved": fl_plain / (kp_ms / 1e3) / 1e12, "peak": PEAK_FP32 / 1e12,
                           "unit": "TFLOP/s", "frac": fl_plain / (kp_ms / 1e3) / PEAK_FP32,
                           "kernel": "k_tx + k_rx (FP32 VALU)", "kernel_ms_total": kp_ms, "launches": kp_n,
-                          # 32 of the flops per evaluation are useful, but the kernels issue 35.8 VALU
+                          # 32 of the flops per evaluation are useful, but the kernels issue 31.9 VALU
                           # instructions per evaluation at n_sub = 2 (k_tx 15.1 with two frequency
-                          # blocks per thread + k_rx 20.7, counted with SQ_INSTS_VALU), so their bound
+                          # blocks per thread + k_rx 16.8, counted with SQ_INSTS_VALU), so their bound
                           # is the VALU issue rate: 256 CUs x 128 lanes per cycle (two wave64
                           # instructions every 4 cycles per SIMD, reached without packed FP32) =
                           # 78.6 T lane-instructions/s at 2.4 GHz

@@ -144,6 +144,63 @@ __device__ __forceinline__ cf green_step_fast(const GeoF& g, double df) {
   return {m * __builtin_amdgcn_cosf(ph), m * __builtin_amdgcn_sinf(ph)};
 }
 
+// ---------------------------------------------------------------- geometry table entries
+// The frequency-independent part of each (scatterer, element, sub-element) term, written once per call
+// by k_geo_table (simus_mfma.hip) in rows of `ept` elements, [s][e][j]: the clipped distance r_c (FP64),
+// (log2 of the amplitude, FP32 r_c) and the per-frequency step (complex64).
+typedef unsigned fs_u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned fs_u32x2 __attribute__((ext_vector_type(2)));
+
+// The NSUB table entries of one (scatterer, element): one vector load each.
+template <int NSUB>
+struct TableTerm {
+  double r[NSUB];
+  float l[NSUB], rf[NSUB];
+  float2 q[NSUB];
+};
+// Buffer resource over one table row (wave-uniform base, the row's size in
+// bytes as its record count: lanes reading past it get zeros, not a fault).
+constexpr int kRawBufferWord3 = 0x00020000;  // raw (untyped) buffer, 32-bit data format
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, kRawBufferWord3);
+}
+
+// The NSUB entries of table row `row` = s * ept (wave-uniform; ept elements per row) for element e,
+// by buffer loads: scalar row address plus the lane's 32-bit byte offset.
+template <int NSUB>
+__device__ __forceinline__ TableTerm<NSUB> load_term(const double* __restrict__ rt, const float* __restrict__ la,
+                                                     const float2* __restrict__ qt, size_t row, int e, int ept) {
+  TableTerm<NSUB> t;
+  const __amdgpu_buffer_rsrc_t rr = row_rsrc(rt + row * NSUB, ept * NSUB * 8);
+  const __amdgpu_buffer_rsrc_t rl = row_rsrc(la + 2 * row * NSUB, ept * NSUB * 8);
+  const __amdgpu_buffer_rsrc_t rq = row_rsrc(qt + row * NSUB, ept * NSUB * 8);
+  if constexpr (NSUB == 2) {
+    const fs_u32x4 r = __builtin_bit_cast(fs_u32x4, __builtin_amdgcn_raw_buffer_load_b128(rr, e * 16, 0, 0));
+    const fs_u32x4 l = __builtin_bit_cast(fs_u32x4, __builtin_amdgcn_raw_buffer_load_b128(rl, e * 16, 0, 0));
+    const fs_u32x4 q = __builtin_bit_cast(fs_u32x4, __builtin_amdgcn_raw_buffer_load_b128(rq, e * 16, 0, 0));
+    t.r[0] = __builtin_bit_cast(double, (fs_u32x2){r.x, r.y});
+    t.r[1] = __builtin_bit_cast(double, (fs_u32x2){r.z, r.w});
+    t.l[0] = __uint_as_float(l.x);
+    t.rf[0] = __uint_as_float(l.y);
+    t.l[1] = __uint_as_float(l.z);
+    t.rf[1] = __uint_as_float(l.w);
+    t.q[0] = make_float2(__uint_as_float(q.x), __uint_as_float(q.y));
+    t.q[1] = make_float2(__uint_as_float(q.z), __uint_as_float(q.w));
+  } else {
+#pragma unroll
+    for (int j = 0; j < NSUB; ++j) {
+      const int o = (e * NSUB + j);
+      t.r[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rr, o * 8, 0, 0));
+      const fs_u32x2 l = __builtin_bit_cast(fs_u32x2, __builtin_amdgcn_raw_buffer_load_b64(rl, o * 8, 0, 0));
+      t.l[j] = __uint_as_float(l.x);
+      t.rf[j] = __uint_as_float(l.y);
+      const fs_u32x2 q = __builtin_bit_cast(fs_u32x2, __builtin_amdgcn_raw_buffer_load_b64(rq, o * 8, 0, 0));
+      t.q[j] = make_float2(__uint_as_float(q.x), __uint_as_float(q.y));
+    }
+  }
+  return t;
+}
+
 // Split-bf16 form of a pair of FP32 values: hi = bf16(a) | bf16(b) << 16 (round to nearest even) and
 // lo = the same of the remainders a - hi_a, b - hi_b (exact in FP32).
 typedef __bf16 fs_bf16x2 __attribute__((ext_vector_type(2)));

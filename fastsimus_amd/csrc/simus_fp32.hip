@@ -20,7 +20,12 @@
 //  * k_rx: lane = element, scatterers are walked uniformly, so Q(f, s) is a
 //    scalar load; the 32 X(f, e) accumulators live in VGPRs; every wave writes a
 //    private partial sum (no atomics) and k_reduce adds them in a fixed order,
-//    so results are bitwise reproducible.
+//    so results are bitwise reproducible. Where the amplitude is non-negative
+//    (centre-frequency directivity, up to 4 sub-elements) the frequency-
+//    independent part of each term (clipped distance, log2 amplitude, step) is
+//    read from a geometry table written once per call and shared by all
+//    transmits, so a block's seed is one FP64 phase reduction, one exp2 and one
+//    sine / cosine pair per term.
 #include <algorithm>
 
 #include "fs_device.h"
@@ -196,10 +201,15 @@ __global__ void k_pfield_reduce(const float* __restrict__ pw, int nfb, int S, fl
   out[s] = sqrtf(acc * corr);
 }
 
-// part[(chunk * 4 + wave)][k - k_begin][e], k_begin = first frequency of the block group
-template <int NSUB, bool FULLDIR, bool GROUPED = false>
+// part[(chunk * 4 + wave)][k - k_begin][e], k_begin = first frequency of the block group.
+// TAB: the frequency-independent part of every term (clipped distance, log2 of the amplitude, the step)
+// comes from the geometry table [s][e][j] (rows of p.E elements), so that a block's seed costs one FP64
+// phase reduction, one exp2 and one sine / cosine pair per term instead of the FP64 geometry.
+template <int NSUB, bool FULLDIR, bool GROUPED = false, bool TAB = false>
 __global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_per_wave, const cf* __restrict__ Q,
-                                            cf* __restrict__ part) {
+                                            cf* __restrict__ part, const double* __restrict__ rt,
+                                            const float* __restrict__ la, const float2* __restrict__ qt) {
+  static_assert(!TAB || (!FULLDIR && !GROUPED), "the table holds centre-frequency directivity, up to 4 sub-elements");
   const int lane = threadIdx.x & 63;
   // wave-uniform by construction; readfirstlane tells the compiler, so that the
   // scatterer loop is a scalar loop and Q, x, z, rc are scalar loads
@@ -216,6 +226,8 @@ __global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_
   for (int k = 0; k < FB; ++k) X[k] = {0.f, 0.f};
 
   const double fstart = p.f0 + (double)k0 * p.df;
+  const double nu0 = fstart * p.inv_c;                                     // revolutions per metre at fstart
+  const float a0 = (float)((double)p.att_coef * (double)kLog2e * fstart);  // log2 attenuation per metre
   const int sb = (chunk * 4 + wave) * s_per_wave;
   const int se = min(sb + s_per_wave, p.S);
   const cf* Qb = Q + (size_t)blockIdx.y * p.S * FB;
@@ -238,6 +250,16 @@ __global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_
     for (int jg = 0; jg < ngroups; ++jg) {
     cf g[NSUB], q[NSUB];
     GeoF geo[NSUB];
+    if constexpr (TAB) {
+      const TableTerm<NSUB> tt = load_term<NSUB>(rt, la, qt, (size_t)s * p.E, ee, p.E);
+#pragma unroll
+      for (int j = 0; j < NSUB; ++j) {
+        const float ph = (float)__builtin_amdgcn_fract(nu0 * tt.r[j]);  // v_fract_f64: x - floor(x)
+        const float m = __builtin_amdgcn_exp2f(fmaf(-a0, tt.rf[j], tt.l[j]));
+        g[j] = {m * __builtin_amdgcn_cosf(ph), m * __builtin_amdgcn_sinf(ph)};
+        q[j] = {tt.q[j].x, tt.q[j].y};
+      }
+    } else {
 #pragma unroll
     for (int j = 0; j < NSUB; ++j) {
       if constexpr (GROUPED) {
@@ -249,6 +271,7 @@ __global__ __launch_bounds__(256) void k_rx(Problem p, int nbuf, int fb0, int s_
       }
       g[j] = green_fast(geo[j], fstart, geo[j].amp);
       q[j] = green_step_fast(geo[j], p.df);
+    }
     }
 #pragma unroll
     for (int k = 0; k < FB; ++k) {
@@ -314,8 +337,17 @@ __global__ __launch_bounds__(64 * RG) void k_reduce(Problem p, int nbuf, int npa
 struct Fp32Plan {
   int nfp, nfb, fbg, chunks, s_per_wave, nparts;
   bool table_per_tx;  // the TX table holds one transmit, rebuilt before each (else all transmits at once)
-  size_t off_A, off_pp, off_pr, off_Q, off_part, total;
+  bool geo;           // k_rx reads the geometry table (built once per call, shared by all transmits)
+  size_t off_A, off_pp, off_pr, off_Q, off_part, off_scat, off_rt, off_la, off_qt, total;
 };
+
+bool plain_geo_table_possible(const Problem& p);  // simus_mfma.hip
+void launch_plain_geo_table(const Problem& p, float4* scat, double* rt, float* la, float2* qt, hipStream_t st);
+
+// Geometry table of k_rx: (S + 1) x E x n_sub terms of 24 B (plus 16 B per scatterer of inputs), used by
+// the default plan up to this size.
+constexpr size_t kPlainGeoTableMax = (size_t)16 << 30;
+static size_t plain_geo_bytes(const Problem& p) { return (size_t)(p.S + 1) * p.E * p.NS * 24 + (size_t)p.S * 16; }
 
 static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
@@ -331,10 +363,11 @@ static int default_chunks(const Problem& p, int fbg) {
   return std::min(chunks, max_chunks);
 }
 
-static void fp32_layout(const Problem& p, int fbg, int chunks, bool table_per_tx, Fp32Plan& P) {
+static void fp32_layout(const Problem& p, int fbg, int chunks, bool table_per_tx, bool geo, Fp32Plan& P) {
   P.fbg = fbg;
   P.chunks = chunks;
   P.table_per_tx = table_per_tx;
+  P.geo = geo;
   P.s_per_wave = (p.S + chunks * 4 - 1) / (chunks * 4);
   P.nparts = chunks * 4;
   const size_t per_block = (size_t)p.S * FB * sizeof(cf);
@@ -344,6 +377,14 @@ static void fp32_layout(const Problem& p, int fbg, int chunks, bool table_per_tx
   P.off_pr = off;   off = align_up(off + (size_t)P.nfp * sizeof(float));
   P.off_Q = off;    off = align_up(off + per_block * P.fbg);
   P.off_part = off; off = align_up(off + (size_t)P.nparts * P.fbg * FB * p.E * sizeof(cf));
+  P.off_scat = P.off_rt = P.off_la = P.off_qt = off;
+  if (geo) {
+    const size_t terms = (size_t)(p.S + 1) * p.E * p.NS;
+    P.off_scat = off; off = align_up(off + (size_t)p.S * sizeof(float4));
+    P.off_rt = off;   off = align_up(off + terms * sizeof(double));
+    P.off_la = off;   off = align_up(off + terms * 2 * sizeof(float));
+    P.off_qt = off;   off = align_up(off + terms * sizeof(float2));
+  }
   P.total = off;
 }
 
@@ -356,7 +397,8 @@ Fp32Plan fp32_plan(const Problem& p) {
   const size_t per_block = (size_t)p.S * FB * sizeof(cf);
   const int fbg =
       (int)std::max<size_t>(1, std::min<size_t>((size_t)P.nfb, ((size_t)2 << 30) / std::max<size_t>(per_block, 1)));
-  fp32_layout(p, fbg, default_chunks(p, fbg), false, P);
+  const bool geo = plain_geo_table_possible(p) && plain_geo_bytes(p) <= kPlainGeoTableMax;
+  fp32_layout(p, fbg, default_chunks(p, fbg), false, geo, P);
   return P;
 }
 
@@ -370,18 +412,18 @@ bool fp32_plan_within(const Problem& p, size_t budget, Fp32Plan* out) {
   P.nfp = P.nfb * FB;
   const Fp32Plan def = fp32_plan(p);
   for (int fbg = def.fbg; fbg >= 1; --fbg) {
-    fp32_layout(p, fbg, default_chunks(p, fbg), true, P);
+    fp32_layout(p, fbg, default_chunks(p, fbg), true, false, P);
     if (P.total <= budget) {
       *out = P;
       return true;
     }
   }
-  fp32_layout(p, 1, 1, true, P);
+  fp32_layout(p, 1, 1, true, false, P);
   if (P.total > budget) return false;
   const size_t per_chunk = (size_t)4 * FB * p.E * sizeof(cf);
   const int extra = (int)std::min<size_t>((budget - P.total) / per_chunk, (size_t)default_chunks(p, 1) - 1);
-  fp32_layout(p, 1, 1 + extra, true, P);
-  while (P.total > budget && P.chunks > 1) fp32_layout(p, 1, P.chunks - 1, true, P);  // alignment slack
+  fp32_layout(p, 1, 1 + extra, true, false, P);
+  while (P.total > budget && P.chunks > 1) fp32_layout(p, 1, P.chunks - 1, true, false, P);  // alignment slack
   *out = P;
   return P.total <= budget;
 }
@@ -411,23 +453,34 @@ static void launch_tx(const Problem& p, int tx, const Fp32Plan& P, int fb0, int 
   timing_stop(slot, st);
 }
 template <int NSUB, bool FD, bool GR>
-static void launch_rx(const Problem& p, const Fp32Plan& P, int fb0, int nb, const cf* Q, cf* part, hipStream_t st) {
+static void launch_rx(const Problem& p, const Fp32Plan& P, int fb0, int nb, const cf* Q, cf* part, const char* ws,
+                      hipStream_t st) {
   dim3 grid((p.E + 63) / 64, nb, P.chunks);
   const int slot = timing_start(st);
-  hipLaunchKernelGGL((k_rx<NSUB, FD, GR>), grid, dim3(256), 0, st, p, P.fbg * FB, fb0, P.s_per_wave, Q, part);
+  if constexpr (!FD && !GR) {
+    if (P.geo) {
+      hipLaunchKernelGGL((k_rx<NSUB, false, false, true>), grid, dim3(256), 0, st, p, P.fbg * FB, fb0, P.s_per_wave,
+                         Q, part, (const double*)(ws + P.off_rt), (const float*)(ws + P.off_la),
+                         (const float2*)(ws + P.off_qt));
+      timing_stop(slot, st);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_rx<NSUB, FD, GR>), grid, dim3(256), 0, st, p, P.fbg * FB, fb0, P.s_per_wave, Q, part,
+                     (const double*)nullptr, (const float*)nullptr, (const float2*)nullptr);
   timing_stop(slot, st);
 }
 
 // NSUB sub-elements exactly, or (GR) any count in groups of NSUB
 template <int NSUB, bool GR = false>
 static int dispatch_fd(const Problem& p, int tx, const Fp32Plan& P, int fb0, int nb, const cf* A, const cf* pp, cf* Q,
-                       cf* part, hipStream_t st) {
+                       cf* part, const char* ws, hipStream_t st) {
   if (p.fulldir) {
     launch_tx<NSUB, true, GR>(p, tx, P, fb0, nb, A, pp, Q, st);
-    launch_rx<NSUB, true, GR>(p, P, fb0, nb, Q, part, st);
+    launch_rx<NSUB, true, GR>(p, P, fb0, nb, Q, part, ws, st);
   } else {
     launch_tx<NSUB, false, GR>(p, tx, P, fb0, nb, A, pp, Q, st);
-    launch_rx<NSUB, false, GR>(p, P, fb0, nb, Q, part, st);
+    launch_rx<NSUB, false, GR>(p, P, fb0, nb, Q, part, ws, st);
   }
   return FS_OK;
 }
@@ -440,16 +493,16 @@ static void launch_tx_table_n(const Problem& p, int nfp, cf* A, hipStream_t st) 
 
 // All frequency-block groups of transmit `tx` of p (its TX table at A + tx E nfp), reduced into spect.
 static int fp32_transmit(const Problem& p, const Fp32Plan& P, int tx, const cf* A, const cf* pp, const float* pr, cf* Q,
-                         cf* part, cf* spect, hipStream_t st) {
+                         cf* part, cf* spect, const char* ws, hipStream_t st) {
   for (int fb0 = 0; fb0 < P.nfb; fb0 += P.fbg) {
     const int nb = std::min(P.fbg, P.nfb - fb0);
     int rc;
     switch (p.NS) {
-      case 1: rc = dispatch_fd<1>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
-      case 2: rc = dispatch_fd<2>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
-      case 3: rc = dispatch_fd<3>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
-      case 4: rc = dispatch_fd<4>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
-      default: rc = dispatch_fd<4, true>(p, tx, P, fb0, nb, A, pp, Q, part, st); break;
+      case 1: rc = dispatch_fd<1>(p, tx, P, fb0, nb, A, pp, Q, part, ws, st); break;
+      case 2: rc = dispatch_fd<2>(p, tx, P, fb0, nb, A, pp, Q, part, ws, st); break;
+      case 3: rc = dispatch_fd<3>(p, tx, P, fb0, nb, A, pp, Q, part, ws, st); break;
+      case 4: rc = dispatch_fd<4>(p, tx, P, fb0, nb, A, pp, Q, part, ws, st); break;
+      default: rc = dispatch_fd<4, true>(p, tx, P, fb0, nb, A, pp, Q, part, ws, st); break;
     }
     if (rc != FS_OK) return rc;
     const int kb = fb0 * FB, ke = std::min((fb0 + nb) * FB, p.NF);
@@ -466,10 +519,13 @@ int fp32_spectrum_plan(const Problem& p, const Fp32Plan& P, cf* spect, char* ws,
   cf* Q = (cf*)(ws + P.off_Q);
   cf* part = (cf*)(ws + P.off_part);
   hipLaunchKernelGGL(k_spectra, dim3((P.nfp + 255) / 256), dim3(256), 0, st, p, P.nfp, pp, pr);
+  if (P.geo)
+    launch_plain_geo_table(p, (float4*)(ws + P.off_scat), (double*)(ws + P.off_rt), (float*)(ws + P.off_la),
+                           (float2*)(ws + P.off_qt), st);
   if (!P.table_per_tx) {
     launch_tx_table_n(p, P.nfp, A, st);
     for (int tx = 0; tx < p.NT; ++tx) {
-      const int rc = fp32_transmit(p, P, tx, A, pp, pr, Q, part, spect, st);
+      const int rc = fp32_transmit(p, P, tx, A, pp, pr, Q, part, spect, ws, st);
       if (rc != FS_OK) return rc;
     }
   } else {
@@ -480,7 +536,7 @@ int fp32_spectrum_plan(const Problem& p, const Fp32Plan& P, cf* spect, char* ws,
       pt.delay = p.delay + (size_t)tx * p.E;
       pt.apod = p.apod + (size_t)tx * p.E;
       launch_tx_table_n(pt, P.nfp, A, st);
-      const int rc = fp32_transmit(pt, P, 0, A, pp, pr, Q, part, spect + (size_t)tx * p.NF * p.E, st);
+      const int rc = fp32_transmit(pt, P, 0, A, pp, pr, Q, part, spect + (size_t)tx * p.NF * p.E, ws, st);
       if (rc != FS_OK) return rc;
     }
   }

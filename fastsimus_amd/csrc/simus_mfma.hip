@@ -259,6 +259,16 @@ __global__ void k_scat_prep(Problem p, float4* __restrict__ out) {
   out[s] = make_float4(x, z, w, 0.f);
 }
 
+// Per-scatterer inputs of the plain kernels' geometry table: (x, z, 1 / n_sub, 0), the weight 0 for rc = 0
+// and for scatterers out of the field (the plain kernels apply rc themselves).
+__global__ void k_scat_prep_unit(Problem p, float4* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= p.S) return;
+  const float x = p.x[s], z = p.z[s];
+  const bool live = p.rc[s] != 0.f && !out_of_field(p, x, z);
+  out[s] = make_float4(x, z, live ? p.inv_nsub : 0.f, 0.f);
+}
+
 // ---------------------------------------------------------------- geometry table
 // Frequency-independent part of every (scatterer, element, sub-element) term,
 // evaluated once per call (reference math: _pfield_math.py:71-247); the distance
@@ -415,56 +425,6 @@ __device__ __forceinline__ void gen_fragments(const Problem& p, const ElemData<N
     frag[L::idx(k, 0, tile, lane)] = (u32x4){h0, h1, h2, h3};
     frag[L::idx(k, 1, tile, lane)] = (u32x4){l0, l1, l2, l3};
   }
-}
-
-// The NSUB table entries of one (scatterer, element): one vector load each.
-template <int NSUB>
-struct TableTerm {
-  double r[NSUB];
-  float l[NSUB], rf[NSUB];
-  float2 q[NSUB];
-};
-// Buffer resource over one table row (wave-uniform base, the row's size in
-// bytes as its record count: lanes reading past it get zeros, not a fault).
-constexpr int kRawBufferWord3 = 0x00020000;  // raw (untyped) buffer, 32-bit data format
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* base, int bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, kRawBufferWord3);
-}
-
-// The NSUB entries of table row `row` = s * ept (wave-uniform; ept elements per row) for element e,
-// by buffer loads: scalar row address plus the lane's 32-bit byte offset.
-template <int NSUB>
-__device__ __forceinline__ TableTerm<NSUB> load_term(const double* __restrict__ rt, const float* __restrict__ la,
-                                                     const float2* __restrict__ qt, size_t row, int e, int ept) {
-  TableTerm<NSUB> t;
-  const __amdgpu_buffer_rsrc_t rr = row_rsrc(rt + row * NSUB, ept * NSUB * 8);
-  const __amdgpu_buffer_rsrc_t rl = row_rsrc(la + 2 * row * NSUB, ept * NSUB * 8);
-  const __amdgpu_buffer_rsrc_t rq = row_rsrc(qt + row * NSUB, ept * NSUB * 8);
-  if constexpr (NSUB == 2) {
-    const u32x4 r = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rr, e * 16, 0, 0));
-    const u32x4 l = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rl, e * 16, 0, 0));
-    const u32x4 q = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rq, e * 16, 0, 0));
-    t.r[0] = __builtin_bit_cast(double, (u32x2){r.x, r.y});
-    t.r[1] = __builtin_bit_cast(double, (u32x2){r.z, r.w});
-    t.l[0] = __uint_as_float(l.x);
-    t.rf[0] = __uint_as_float(l.y);
-    t.l[1] = __uint_as_float(l.z);
-    t.rf[1] = __uint_as_float(l.w);
-    t.q[0] = make_float2(__uint_as_float(q.x), __uint_as_float(q.y));
-    t.q[1] = make_float2(__uint_as_float(q.z), __uint_as_float(q.w));
-  } else {
-#pragma unroll
-    for (int j = 0; j < NSUB; ++j) {
-      const int o = (e * NSUB + j);
-      t.r[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rr, o * 8, 0, 0));
-      const u32x2 l = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(rl, o * 8, 0, 0));
-      t.l[j] = __uint_as_float(l.x);
-      t.rf[j] = __uint_as_float(l.y);
-      const u32x2 q = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(rq, o * 8, 0, 0));
-      t.q[j] = make_float2(__uint_as_float(q.x), __uint_as_float(q.y));
-    }
-  }
-  return t;
 }
 
 // One thread's table entries for one tile (its 4 scatterers).
@@ -1370,6 +1330,22 @@ int mfma_spectrum_batch(const Problem& p, int P, const int64_t* off, cf* spect, 
   else
     hipLaunchKernelGGL(k_msm_apply<40>, ag, dim3(256), 0, st, p, m.ET * 16, m.phantoms, m.nfpad, kpart, spect);
   return hipGetLastError() == hipSuccess ? FS_OK : FS_ERR_LAUNCH;
+}
+
+// Geometry table of the plain kernels (k_rx): rows of p.E elements (no padding), amplitude without
+// sqrt|rc|, for 1 to 4 sub-elements, a non-negative amplitude and centre-frequency directivity.
+bool plain_geo_table_possible(const Problem& p) { return p.NS >= 1 && table_possible(p); }
+
+void launch_plain_geo_table(const Problem& p, float4* scat, double* rt, float* la, float2* qt, hipStream_t st) {
+  if (p.S > 0) hipLaunchKernelGGL(k_scat_prep_unit, dim3((p.S + 255) / 256), dim3(256), 0, st, p, scat);
+  const long long n = (long long)(p.S + 1) * p.E;
+  const dim3 grid((unsigned)((n + 255) / 256));
+  switch (p.NS) {
+    case 1: hipLaunchKernelGGL(k_geo_table<1>, grid, dim3(256), 0, st, p, scat, p.E, rt, la, qt); break;
+    case 2: hipLaunchKernelGGL(k_geo_table<2>, grid, dim3(256), 0, st, p, scat, p.E, rt, la, qt); break;
+    case 3: hipLaunchKernelGGL(k_geo_table<3>, grid, dim3(256), 0, st, p, scat, p.E, rt, la, qt); break;
+    default: hipLaunchKernelGGL(k_geo_table<4>, grid, dim3(256), 0, st, p, scat, p.E, rt, la, qt); break;
+  }
 }
 
 int mfma_spectrum(const Problem& p, cf* spect, char* ws, size_t ws_bytes, hipStream_t st) {
