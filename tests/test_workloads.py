@@ -1,5 +1,9 @@
-"""The three GPU workloads of the benchmark configurations at their full size (marked ``gpu`` and ``slow``).
+"""The GPU workloads of the benchmark configurations at their full size (marked ``gpu`` and ``slow``).
 
+* Single transmit, 128-element 7.6 MHz linear array: 2 000 speckle scatterers with zero delays, both methods
+  against the FP64 oracle on every scatterer; 50 000 speckle scatterers plus 10 bright points under a
+  0 degree plane wave, both methods on the full set against each other and against the FP64 oracle on a
+  1 010-scatterer sample (1 000 speckle + the 10 points) with the full set's plan.
 * Plane-wave compounding: 200 000 speckle scatterers, 75 plane waves in [-16, 16] deg, 128-element 7.6 MHz
   linear array. Split-BF16 against plain FP32 on 3 transmits; both against the FP64 oracle on a
   1 000-scatterer sample of 2 transmits; superposition in rc and the +-angle mirror symmetry at full size.
@@ -39,6 +43,45 @@ def _oracle_plan(plan, params):
             "n_sub": plan.n_sub, "seg": plan.seg_length, "pulse": np.asarray(plan.pulse_spectrum),
             "probe": np.asarray(plan.probe_spectrum), "n_ff": plan.n_freq_full, "idx0": plan.freq_idx_start,
             "corr": plan.correction_factor, "n_fft": plan.n_fft}
+
+
+# ----------------------------------------------------------------------------- single transmit
+def test_single_2000_scatterers_against_fp64_in_full(cuda):
+    params, medium, (x, z, rc, d), _ = bench.single_workloads()
+    sc = np.stack([x, z], -1)
+    plan = fs.simus_precompute(sc, rc, d, params, medium)
+    assert (x.size, plan.n_sub) == (2000, 2)
+    print("single 2 000: FP64 oracle on every scatterer", flush=True)
+    want, _, _ = O.simus_oracle(x.astype(np.float64), z.astype(np.float64), rc.astype(np.float64), d, params, medium,
+                                _oracle_plan(plan, params))
+    for strat, tol in (("hip", 1e-4), ("hip_mfma", 1e-3)):
+        got = fs.simus_compute(sc, rc, d, plan, params, medium, strategy=strat).rf.cpu().numpy()
+        err = rel_l2(got, want)
+        print(f"  {strat} vs FP64 rel-L2 {err:.2e}", flush=True)
+        assert err <= tol, (strat, err)
+
+
+def test_single_50010_scatterers_methods_and_fp64_sample(cuda):
+    params, medium, _, (x, z, rc, d) = bench.single_workloads()
+    sc = np.stack([x, z], -1)
+    plan = fs.simus_precompute(sc, rc, d, params, medium)
+    assert (x.size, plan.n_sub) == (50_010, 2)
+    print("single 50 010: both methods on the full set", flush=True)
+    rf_p = fs.simus_compute(sc, rc, d, plan, params, medium, strategy="hip").rf.cpu().numpy()
+    rf_m = fs.simus_compute(sc, rc, d, plan, params, medium, strategy="hip_mfma").rf.cpu().numpy()
+    e = rel_l2(rf_m, rf_p)
+    print(f"  split-BF16 vs plain FP32 rel-L2 {e:.2e}", flush=True)
+    assert e <= 1e-3
+    assert np.abs(rf_p).max() > 0.0
+    idx = np.concatenate([np.sort(np.random.default_rng(3).choice(50_000, 1000, replace=False)),
+                          np.arange(50_000, 50_010)])
+    want, _, _ = O.simus_oracle(x[idx].astype(np.float64), z[idx].astype(np.float64), rc[idx].astype(np.float64), d,
+                                params, medium, _oracle_plan(plan, params))
+    for strat, tol in (("hip", 1e-4), ("hip_mfma", 1e-3)):
+        got = fs.simus_compute(sc[idx], rc[idx], d, plan, params, medium, strategy=strat).rf.cpu().numpy()
+        err = rel_l2(got, want)
+        print(f"  {strat} vs FP64 (1 010-scatterer sample) rel-L2 {err:.2e}", flush=True)
+        assert err <= tol, (strat, err)
 
 
 # ----------------------------------------------------------------------------- plane-wave compounding
