@@ -421,7 +421,7 @@ __global__ void k_ppc(Problem p, int nfp, cf* __restrict__ ppc) {
 // result layout of W^T = R L^T, so that G, E_j and W of one term sit in the
 // same lane. Y is summed over the waves through LDS.
 constexpr int AM_FC = 32;    // frequencies per chunk = the FP64 reseed interval of the recurrence (3 k + 2)
-constexpr int AM_CPI = 4;    // chunks per work item (the per-item geometry and gradient coefficients are shared)
+constexpr int AM_CPI = 8;    // chunks per work item (the per-item geometry and gradient coefficients are shared)
 constexpr int AM_TG = 16;    // transmits per group
 constexpr int AM_MAXW = 8;   // element slices per workgroup (E <= 128)
 constexpr int AM_NSUB_MAX = 2;
