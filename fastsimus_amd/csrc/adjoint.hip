@@ -426,9 +426,17 @@ constexpr int AM_TG = 16;    // transmits per group
 constexpr int AM_MAXW = 8;   // element slices per workgroup (E <= 128)
 constexpr int AM_NSUB_MAX = 2;
 constexpr int AM_YROW = 16;  // LDS row of Y: 16 scatterers
-// Y rows are stored with their four 4-float groups XOR-swizzled by (row % 16) / 4, so that the
-// 16-byte stores of 16 lanes (16 consecutive rows) and the 4-byte loads of the reduction (64
-// consecutive floats of 4 rows) each fall on 64 distinct banks.
+// Y of one slice is a block of 32 (m) x 16 (s) floats, [x][h][8]: the 4 rows s = 4 g .. 4 g + 3 of column
+// m = 16 x + col (one MFMA result lane, written as one 16-byte store) sit at 4-float slot
+// 8 h + ((col + 2 g) mod 8), h = 4 (col / 8) + g. The LDS has 32 banks of 4 bytes and serves a 16-byte
+// store 8 lanes at a time and a 4-byte load 32 lanes at a time: 8 consecutive lanes (8 columns, one g)
+// land on 8 distinct slots modulo 8, and the 32 lanes of a slice-sum load (two columns, 16 rows) on 32
+// distinct banks, so neither conflicts (the earlier rows of 16 floats with XOR-swizzled groups cost 8
+// conflict cycles per store).
+__device__ __forceinline__ int am_yidx(int m, int s) {
+  const int col = m & 15, g = s >> 2;
+  return (m >> 4) * 256 + (8 * ((col >> 3) * 4 + g) + ((col + 2 * g) & 7)) * 4 + (s & 3);
+}
 
 typedef __bf16 am_bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 am_bf16x2 __attribute__((ext_vector_type(2)));
@@ -583,7 +591,7 @@ __device__ __forceinline__ void am_mfma1(const AmOps& o, const am_u32x4 (&b1)[2]
 // (Re L, -Im L) of its entry c (real part) or (Im L, Re L) (imaginary part), so each entry is one
 // 32-bit word per operand: the sign flip of the upper half or the swap of the halves of its split pair.
 // A lane writes 4 words (hi and lo of both parts); the 64 lanes of a wave write 64 consecutive words.
-// LDS: ypr / ypi [w][m][s] (rows of AM_YROW floats); lfrag [x][part][hl][lane][4 words].
+// LDS: ypr / ypi [w][32 x 16 block, am_yidx]; lfrag [x][part][hl][lane][4 words].
 template <int NW>
 __device__ __forceinline__ void am_reduce(const float* ypr, const float* ypi, unsigned short* lfrag) {
 #pragma unroll
@@ -593,8 +601,8 @@ __device__ __forceinline__ void am_reduce(const float* ypr, const float* ypi, un
     float tr[NW], ti[NW];
 #pragma unroll
     for (int v = 0; v < NW; ++v) {
-      tr[v] = ypr[(v * 32 + m) * AM_YROW + (sl ^ (4 * ((m & 15) >> 2)))];
-      ti[v] = ypi[(v * 32 + m) * AM_YROW + (sl ^ (4 * ((m & 15) >> 2)))];
+      tr[v] = ypr[v * 32 * AM_YROW + am_yidx(m, sl)];
+      ti[v] = ypi[v * 32 * AM_YROW + am_yidx(m, sl)];
     }
     float yx = 0.f, yy = 0.f;
 #pragma unroll
@@ -700,8 +708,8 @@ __global__ __launch_bounds__(64 * NW) void k_adjm(Problem p, int q, int nch, con
     float* ib = ypi + (i & 1) * YB;
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
-      *(am_floatx4*)(yb + (w * 32 + 16 * x + col) * AM_YROW + 4 * (g ^ (col >> 2))) = yr[x];
-      *(am_floatx4*)(ib + (w * 32 + 16 * x + col) * AM_YROW + 4 * (g ^ (col >> 2))) = yi[x];
+      *(am_floatx4*)(yb + w * 32 * AM_YROW + am_yidx(16 * x + col, 4 * g)) = yr[x];
+      *(am_floatx4*)(ib + w * 32 * AM_YROW + am_yidx(16 * x + col, 4 * g)) = yi[x];
     }
   };
   auto stage_a = [&](int i, const cf (&Ei)[4][NSUB]) {
