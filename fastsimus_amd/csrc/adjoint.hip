@@ -428,14 +428,14 @@ constexpr int AM_NSUB_MAX = 2;
 constexpr int AM_YROW = 16;  // LDS row of Y: 16 scatterers
 // Y of one slice is a block of 32 (m) x 16 (s) floats, [x][h][8]: the 4 rows s = 4 g .. 4 g + 3 of column
 // m = 16 x + col (one MFMA result lane, written as one 16-byte store) sit at 4-float slot
-// 8 h + ((col + 2 g) mod 8), h = 4 (col / 8) + g. The LDS has 32 banks of 4 bytes and serves a 16-byte
-// store 8 lanes at a time and a 4-byte load 32 lanes at a time: 8 consecutive lanes (8 columns, one g)
-// land on 8 distinct slots modulo 8, and the 32 lanes of a slice-sum load (two columns, 16 rows) on 32
-// distinct banks, so neither conflicts (the earlier rows of 16 floats with XOR-swizzled groups cost 8
-// conflict cycles per store).
+// 8 h + ((col + 4 (g mod 2)) mod 8), h = 4 (col / 8) + g. The LDS has 32 banks of 4 bytes and serves a
+// 16-byte store 8 lanes at a time and a 4-byte access 32 lanes at a time: 8 consecutive lanes (8 columns,
+// one g) land on 8 distinct slots modulo 8, and each 32 lanes of the slice sum (am_reduce: 4 consecutive
+// columns x 8 rows) on 32 distinct banks, for its loads of Y as for its stores of L, so none of them
+// conflicts (rows of 16 floats with XOR-swizzled groups cost 8 conflict cycles per 16-byte store).
 __device__ __forceinline__ int am_yidx(int m, int s) {
   const int col = m & 15, g = s >> 2;
-  return (m >> 4) * 256 + (8 * ((col >> 3) * 4 + g) + ((col + 2 * g) & 7)) * 4 + (s & 3);
+  return (m >> 4) * 256 + (8 * ((col >> 3) * 4 + g) + ((col + 4 * (g & 1)) & 7)) * 4 + (s & 3);
 }
 
 typedef __bf16 am_bf16x8 __attribute__((ext_vector_type(8)));
@@ -596,8 +596,10 @@ template <int NW>
 __device__ __forceinline__ void am_reduce(const float* ypr, const float* ypi, unsigned short* lfrag) {
 #pragma unroll
   for (int r = 0; r < 512 / (64 * NW); ++r) {
-    const int ent = threadIdx.x + r * 64 * NW;
-    const int m = ent >> 4, sl = ent & 15;
+    // entry (m, sl) of thread t: each 32 threads take 4 consecutive m and 8 of the 16 rows, so that their L
+    // stores (4 words of 8 lanes) and Y loads fall on 32 distinct banks
+    const int t = threadIdx.x + r * 64 * NW;
+    const int m = 4 * (t >> 6) + ((t >> 3) & 3), sl = 8 * ((t >> 5) & 1) + (t & 7);
     float tr[NW], ti[NW];
 #pragma unroll
     for (int v = 0; v < NW; ++v) {
