@@ -87,13 +87,15 @@ def msm_flops_per_launch(n_scat: int, n_elem: int, n_freq: int, fbk: int = 4) ->
 
 
 TRAFFIC_PROFILE = ROOT / "profiles" / "k_msm_planewave75_200k_traffic.json"
+ADJM_TRAFFIC_PROFILE = ROOT / "profiles" / "k_adjm_adjoint15_traffic.json"
 
 
-def measured_traffic(n_scat: int, n_tx: int, n_freq: int, n_elem: int):
-    """HBM bytes per k_msm launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
-    (calibrated on a 1 GiB copy), when they were taken on this very workload; else None."""
+def measured_traffic(n_scat: int, n_tx: int, n_freq: int, n_elem: int, profile=TRAFFIC_PROFILE):
+    """HBM bytes per launch of the profiled kernel (k_msm, or k_adjm with ADJM_TRAFFIC_PROFILE) from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes (calibrated on a 1 GiB copy), when they were taken on
+    this very workload; else None."""
     try:
-        d = json.loads(TRAFFIC_PROFILE.read_text())
+        d = json.loads(profile.read_text())
     except (OSError, ValueError):
         return None
     if d.get("workload") != {"n_scat": n_scat, "n_tx": n_tx, "n_freq": n_freq, "n_elem": n_elem}:
@@ -334,7 +336,8 @@ def bench_adjoint(args) -> int:
         # W = sum_t (two complex MACs per (t, s, e, f))
         useful = 32 * n_tx * S * E * nf
         roofline = {"bound": "mfma", "achieved": useful / (k_ms * 1e-3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
-                    "frac": useful / (k_ms * 1e-3) / 2.5e15, "traffic": None,
+                    "frac": useful / (k_ms * 1e-3) / 2.5e15,
+                    "traffic": measured_traffic(S, n_tx, nf, E, ADJM_TRAFFIC_PROFILE),
                     "kernel": "k_adjm (split-BF16 v_mfma_f32_16x16x32_bf16 + FP32 VALU terms)",
                     "kernel_ms": k_ms, "launches": k_launches, "flops_per_launch": useful / groups,
                     "flops_definition": "32 n_tx S E n_f useful flops (P_t, B_t and the transmit sum of W: two "
