@@ -315,6 +315,44 @@ def test_matrix_adjoint_matches_vector_adjoint(case, cuda):
         assert float((rc * d_rc).sum()) == pytest.approx(L, rel=1e-3)
 
 
+@pytest.mark.gpu
+def test_kernel_timing_classes(cuda):
+    """fs_set_kernel_timing masks: the spectrum class records the k_msm launch of the matrix-core spectrum and
+    nothing of the adjoint, the adjoint class one k_adjm launch per group of 16 transmits (20 transmits: 2) or
+    one k_adj_tx + k_adj_pos pair per transmit, and 0 records nothing."""
+    from fastsimus_amd import _lib
+    from fastsimus_amd import kernels as K
+
+    p, med, x, z, rc, D, nsub = _adjoint_case("l11_soft_t20")
+    sc = np.stack([x, z], -1)
+    plan = fs.simus_precompute(sc, rc, D, p, med, element_splitting=nsub)
+    prob = K.Problem(sc, rc, D, None, p, med, n_sub=plan.n_sub, seg_length=plan.seg_length, freqs=plan.selected_freqs,
+                     pulse=plan.pulse_spectrum, probe=plan.probe_spectrum, correction=plan.correction_factor,
+                     full_frequency_directivity=False, device=cuda)
+    assert prob.n_tx == 20
+    g = torch.zeros((prob.n_tx, prob.n_freq, p.n_elements, 2), dtype=torch.float32, device=cuda)
+    lib = _lib.load()
+    try:
+        lib.fs_set_kernel_timing(1)  # FS_TIMING_SPECTRUM
+        _lib.kernel_time_ms()
+        K.run_spectrum(prob, "mfma")
+        _spectral_adjoint(prob, g, "matrix")
+        ms, n = _lib.kernel_time_ms()
+        assert n == 1 and ms > 0.0
+        lib.fs_set_kernel_timing(2)  # FS_TIMING_ADJOINT
+        K.run_spectrum(prob, "mfma")
+        _spectral_adjoint(prob, g, "matrix")
+        ms, n = _lib.kernel_time_ms()
+        assert n == 2 and ms > 0.0
+        _spectral_adjoint(prob, g, "vector")
+        assert _lib.kernel_time_ms()[1] == prob.n_tx
+        lib.fs_set_kernel_timing(0)
+        _spectral_adjoint(prob, g, "matrix")
+        assert _lib.kernel_time_ms()[1] == 0
+    finally:
+        lib.fs_set_kernel_timing(0)
+
+
 def _random_adjoint_case(seed):
     """A random configuration the matrix-core adjoint covers (n_elem <= 128, n_sub <= 2), 1-20 transmits."""
     from math import inf

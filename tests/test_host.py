@@ -329,3 +329,19 @@ def test_batched_workspace_layout(n_elem, n_sub):
     terms = (S + 1) * 64 * groups * n_sub
     table = al(terms * 8) + al(terms * 8) + al(terms * 8) if n_sub <= 4 else 0
     assert got == al(kpart) + al(S * 16) + table + 256
+
+
+def test_bench_traffic_profiles_match_their_workloads():
+    """The bench lines' HBM traffic comes from the committed rocprofv3 passes of exactly their workload
+    (FETCH_SIZE x fetch_scale + WRITE_SIZE x write_scale), and from none for other sizes."""
+    import json
+
+    import bench
+
+    for path, key in ((bench.TRAFFIC_PROFILE, (200_000, 75, 1892, 128)),
+                      (bench.ADJM_TRAFFIC_PROFILE, (65_536, 15, 1711, 128))):
+        d = json.loads(path.read_text())
+        assert (d["workload"]["n_scat"], d["workload"]["n_tx"], d["workload"]["n_freq"], d["workload"]["n_elem"]) == key
+        assert abs(d["fetch_scale"] - 2.0) < 1e-3 and d["write_scale"] == 1.0
+        assert bench.measured_traffic(*key, profile=path) == d["fetch_bytes"] + d["write_bytes"]
+        assert bench.measured_traffic(key[0] + 1, *key[1:], profile=path) is None
