@@ -345,3 +345,36 @@ def test_bench_traffic_profiles_match_their_workloads():
         assert abs(d["fetch_scale"] - 2.0) < 1e-3 and d["write_scale"] == 1.0
         assert bench.measured_traffic(*key, profile=path) == d["fetch_bytes"] + d["write_bytes"]
         assert bench.measured_traffic(key[0] + 1, *key[1:], profile=path) is None
+
+
+@pytest.mark.parametrize("tilt_deg", [0.0, 10.0, -25.0])
+def test_convex_plane_wave_is_the_projection_on_the_direction(tilt_deg):
+    """A plane wave travelling along u = (sin tilt, cos tilt) reaches element p at u . p / c: the convex-array
+    delays (the reference's point-to-line distance, tx_delay.py:50-66) equal (u . p - min u . p) / c."""
+    p = fs.C5_2v()
+    pos, _, _ = fs.element_positions(p.n_elements, p.pitch, p.radius)
+    th = np.deg2rad(tilt_deg)
+    d = fs.plane_wave(pos, th, speed_of_sound=1540.0, radius=p.radius)
+    proj = pos @ np.array([np.sin(th), np.cos(th)])
+    np.testing.assert_allclose(d, (proj - proj.min()) / 1540.0, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("tilt_deg,width_deg", [(0.0, 60.0), (10.0, 40.0), (-20.0, 90.0)])
+def test_diverging_wave_virtual_source_spans_the_width(tilt_deg, width_deg):
+    """The virtual source of a diverging wave sees the two ends of the aperture at tilt -+ width / 2 from the
+    array normal, and the delays are those of that source (a focus behind the array)."""
+    from fastsimus_amd.tx_delay import _angles_to_virtual_source
+
+    q = fs.L11_5v()
+    pos, _, _ = fs.element_positions(128, q.pitch, q.radius)
+    aperture = 127 * q.pitch
+    th, w = np.deg2rad(tilt_deg), np.deg2rad(width_deg)
+    x0, z0 = _angles_to_virtual_source(aperture, th, w)
+    assert z0 < 0
+    a_left = np.arctan2(-aperture / 2 - x0, -z0)
+    a_right = np.arctan2(aperture / 2 - x0, -z0)
+    assert a_right - a_left == pytest.approx(w, abs=1e-12)
+    assert (a_left + a_right) / 2 == pytest.approx(th, abs=1e-12)
+    d = fs.diverging_wave(pos, th, w, aperture_length=aperture, speed_of_sound=1540.0)
+    r = np.hypot(pos[:, 0] - x0, pos[:, 1] - z0)
+    np.testing.assert_allclose(d, (r - r.min()) / 1540.0, rtol=0, atol=1e-15)
