@@ -18,7 +18,7 @@ import fastsimus_amd as fs
 from oracle import simus_oracle as O
 from tests.conftest import f32, rel_l2
 
-SEEDS = list(range(448))
+SEEDS = list(range(1024))
 
 
 def random_case(seed: int):
@@ -95,7 +95,7 @@ def test_random_configuration_matches_oracle(seed, cuda):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("seed", list(range(224)))
+@pytest.mark.parametrize("seed", list(range(512)))
 def test_random_pfield_matches_oracle(seed, cuda):
     """pfield on random arrays, media and grids (points behind the array included) against the FP64 oracle."""
     params, medium, _, _, _, D, kw = random_case(seed)
