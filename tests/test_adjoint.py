@@ -118,8 +118,8 @@ def test_hip_adjoint_matches_fp64_gradient(method, adjoint, cuda):
     tgt = target.to(torch.float32).to(cuda)
     loss = 0.5 * ((rf - tgt) ** 2).sum()
     loss.backward()
-    print("loss", float(loss), L, flush=True)
-    assert float(loss) == pytest.approx(L, rel=1e-3)
+    print("loss", float(loss.detach()), L, flush=True)
+    assert float(loss.detach()) == pytest.approx(L, rel=1e-3)
     assert rel_l2(rt.grad.cpu().numpy(), g_rc) < 1e-3
     assert rel_l2(xt.grad.cpu().numpy(), g_x) < 1e-3
     assert rel_l2(zt.grad.cpu().numpy(), g_z) < 1e-3
