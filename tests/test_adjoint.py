@@ -196,7 +196,7 @@ def test_hip_adjoint_matches_finite_differences(fulldir, cuda):
         loss = (rf.double() * w).sum()
         if grad:
             loss.backward()
-            return float(loss), xt.grad.cpu().numpy(), zt.grad.cpu().numpy(), rt.grad.cpu().numpy(), float(ct.grad)
+            return float(loss.detach()), xt.grad.cpu().numpy(), zt.grad.cpu().numpy(), rt.grad.cpu().numpy(), float(ct.grad)
         return float(loss.detach())
 
     _, gx, gz, grc, gc = run(x, z, rc, 1540.0, grad=True)
