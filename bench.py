@@ -345,6 +345,7 @@ def bench_adjoint(args) -> int:
                     "mfma_issued_flops_per_launch": issued / groups,
                     "mfma_issue_tflops": issued / (k_ms * 1e-3) / 1e12,
                     "mfma_issue_frac": issued / (k_ms * 1e-3) / 2.5e15}
+    cpu = None if args.no_cpu else adjoint_cpu_baseline(params, x, z, rc, rc_t, D, att)
     out = {
         "metric": "G(scatterer·element·freq)/s, forward + adjoint (dL/d rc, x, z, c)", "value": value, "unit": UNIT,
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_step * 1e3,
@@ -356,10 +357,51 @@ def bench_adjoint(args) -> int:
                    "n_sub": plan.n_sub,
                    "method": "hip_mfma forward (multistatic matrix) + hand-written adjoint", "adjoint": args.adjoint},
         f"{other}_adjoint_ms_per_step": t_other * 1e3, f"{other}_adjoint_kernel_ms": k_other,
-        "roofline": roofline, "cpu_baseline": None,
+        "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def adjoint_cpu_baseline(params, x, z, rc, rc_t, D, att, n_sample: int = 32, repeats: int = 3):
+    """The FP64 torch restatement of the differentiable reference path (oracle.simus_grad, the scan strategy's
+    forward with autograd through it) on a bounded sample of the jittered grid: forward RF, L2 loss against the
+    target of the perturbed phantom at c = 1560 m/s, and the gradient w.r.t. rc, x, z and c, all 15 transmits and
+    all frequencies, one thread, best of ``repeats`` (target and plan outside the timing)."""
+    import torch
+
+    import fastsimus_amd as fs
+    from oracle import simus_oracle as O
+    from oracle.simus_grad import simus_rf_torch
+
+    medium = fs.MediumParams(attenuation=att)
+    plan_o = O.simus_plan(x.astype(np.float64), z.astype(np.float64), D, params, medium)
+    idx = np.sort(np.random.default_rng(13).choice(x.size, n_sample, replace=False))
+    xs, zs, rs, rts = (a[idx].astype(np.float64) for a in (x, z, rc, rc_t))
+    threads = torch.get_num_threads()
+    torch.set_num_threads(1)
+    try:
+        with torch.no_grad():
+            target = simus_rf_torch(torch.as_tensor(xs), torch.as_tensor(zs), torch.as_tensor(rts), 1560.0, D,
+                                    params, att, plan_o)
+        times = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            xt, zt, rt = (torch.tensor(a, requires_grad=True) for a in (xs, zs, rs))
+            ct = torch.tensor(1540.0, dtype=torch.float64, requires_grad=True)
+            loss = 0.5 * ((simus_rf_torch(xt, zt, rt, ct, D, params, att, plan_o) - target) ** 2).sum()
+            loss.backward()
+            times.append(time.perf_counter() - t0)
+    finally:
+        torch.set_num_threads(threads)
+    dt = min(times)
+    evals = n_sample * params.n_elements * plan_o["freqs"].size * D.shape[0]
+    return {"value": evals / dt / 1e9, "unit": UNIT, "cores": 1, "kind": "port",
+            "sample": f"{n_sample} of {x.size} scatterers, all {D.shape[0]} transmits and "
+                      f"{plan_o['freqs'].size} frequencies, FP64 torch forward + autograd backward "
+                      f"(oracle.simus_grad), best of {repeats}: {dt:.2f} s (runs {', '.join(f'{t:.2f}' for t in times)} "
+                      f"s), host {platform.processor() or platform.machine()} ({os.cpu_count()} logical CPUs visible); "
+                      "rate per (transmit, scatterer, element, frequency) of forward + gradient"}
 
 
 def single_workloads(seed: int = 2025):
