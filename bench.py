@@ -218,11 +218,22 @@ def bench_phantoms(args) -> int:
     rf_sub = B.simus_batch(sub, beams, params, medium, strategy="hip_mfma").rf.cpu().numpy()
     allp = np.concatenate([p for p, _ in sub]).astype(np.float64)
     plan_o = O.simus_plan(allp[:, 0], allp[:, 1], beams, params, medium)
+    from threadpoolctl import threadpool_limits
+
     errs = []
-    for q, (p_, r_) in enumerate(sub):
-        want = np.stack([O.simus_oracle(p_[:, 0].astype(np.float64), p_[:, 1].astype(np.float64),
-                                        r_.astype(np.float64), d, params, medium, plan_o)[0] for d in beams])
-        errs.append(rel_l2(rf_sub[q], want))
+    t_cpu = 0.0
+    with threadpool_limits(limits=1):
+        for q, (p_, r_) in enumerate(sub):
+            t0 = time.perf_counter()
+            want = np.stack([O.simus_oracle(p_[:, 0].astype(np.float64), p_[:, 1].astype(np.float64),
+                                            r_.astype(np.float64), d, params, medium, plan_o)[0] for d in beams])
+            t_cpu += time.perf_counter() - t0
+            errs.append(rel_l2(rf_sub[q], want))
+    ev_cpu = sum(len(p_) for p_, _ in sub) * E * plan_o["freqs"].size * len(beams)
+    cpu = {"value": ev_cpu / t_cpu / 1e9, "unit": UNIT, "cores": 1, "kind": "port",
+           "sample": f"first 150 scatterers of 6 phantoms x 3 beams, all {plan_o['freqs'].size} frequencies of the "
+                     f"sub-batch plan, FP64 NumPy, {t_cpu:.2f} s in all, host "
+                     f"{platform.processor() or platform.machine()} ({os.cpu_count()} logical CPUs visible)"}
 
     out = {
         "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -245,7 +256,7 @@ def bench_phantoms(args) -> int:
         "tuning": tuning,
         "rf_rel_l2_fp64_per_phantom": {"max": max(errs), "values": errs,
                                        "sample": "first 150 scatterers of phantoms 0-5, beams 0, 64, 127"},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0
