@@ -1057,14 +1057,16 @@ static bool matrix_applies(const Problem& p) { return p.E <= 16 * AM_MAXW && p.N
 
 static bool use_matrix(const Problem& p, int method) { return method != FS_ADJOINT_VECTOR && matrix_applies(p); }
 
+int fill_zero(void* ptr, size_t n_words, hipStream_t st);  // rf.hip
+
 size_t adjoint_workspace(const Problem& p, int method) {
   return use_matrix(p, method) ? adjm_layout(p).total : adj_layout(p).total;
 }
 
 int adjoint_fp32(const Problem& p, int method, const cf* dspect, float* d_rc, float* d_x, float* d_z, double* d_c,
                  char* ws, hipStream_t st) {
-  if (p.S == 0) return FS_ERR_INVALID;
   if (method == FS_ADJOINT_MATRIX && !matrix_applies(p)) return FS_ERR_UNSUPPORTED;
+  if (p.S == 0) return fill_zero(d_c, 2, st);  // no scatterers: the loss does not depend on c
   if (use_matrix(p, method)) return adjoint_matrix(p, dspect, d_rc, d_x, d_z, d_c, ws, st);
   return adjoint_vector(p, dspect, d_rc, d_x, d_z, d_c, ws, st);
 }

@@ -293,8 +293,9 @@ int fs_simus_adjoint(const fs_simus_desc* d, int adjoint_method, const float* d_
   int rc = make_problem(d, &p, true, true);
   if (rc != FS_OK) return rc;
   if (!valid_adjoint_method(adjoint_method)) return FS_ERR_INVALID;
-  if (!d_spect || !d_rc || !d_x || !d_z || !d_c) return FS_ERR_INVALID;
-  if (!workspace || workspace_bytes < adjoint_workspace(p, adjoint_method)) return FS_ERR_WORKSPACE;
+  // without scatterers the per-scatterer outputs are empty (their pointers may be null) and only d_c is written
+  if (!d_c || (p.S > 0 && (!d_spect || !d_rc || !d_x || !d_z))) return FS_ERR_INVALID;
+  if (p.S > 0 && (!workspace || workspace_bytes < adjoint_workspace(p, adjoint_method))) return FS_ERR_WORKSPACE;
   return adjoint_fp32(p, adjoint_method, (const cf*)d_spect, d_rc, d_x, d_z, d_c, (char*)workspace,
                       (hipStream_t)stream);
 }

@@ -186,7 +186,8 @@ enum fs_adjoint_method { FS_ADJOINT_AUTO = 0, FS_ADJOINT_MATRIX = 1, FS_ADJOINT_
 
 /* Gradients of L through fs_simus_spectrum (method FP32 algebra) given dL/d(spect) (the corrected
  * selected spectrum, interleaved complex): d_rc, d_x, d_z [S] float32 (overwritten),
- * d_c [1] float64 (overwritten). Green's functions are recomputed, not stored. The workspace query
+ * d_c [1] float64 (overwritten). Green's functions are recomputed, not stored. With n_scat = 0 only
+ * d_c is written (0; the other pointers and the workspace may be null). The workspace query
  * returns 0 for an invalid descriptor or method. */
 size_t fs_simus_adjoint_workspace_bytes(const fs_simus_desc* d, int adjoint_method);
 int fs_simus_adjoint(const fs_simus_desc* d, int adjoint_method, const float* d_spect, float* d_rc, float* d_x,

@@ -127,6 +127,25 @@ def test_hip_adjoint_matches_fp64_gradient(method, adjoint, cuda):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("method,adjoint", [("fp32", "vector"), ("fp32", "matrix"), ("mfma", "matrix")])
+def test_hip_adjoint_of_an_empty_scene(method, adjoint, cuda):
+    """No scatterers (under the plan of a non-empty scene): the RF is zero, the per-scatterer gradients are
+    empty and dL/dc is 0, since the loss does not depend on c."""
+    from fastsimus_amd.autograd import simus_rf
+
+    x, z, rc, _, D = _scene(10)
+    plan = fs.simus_precompute(np.stack([x, z], -1), rc, D, PARAMS, fs.MediumParams(attenuation=ATT))
+    xt, zt, rt = (torch.zeros(0, dtype=torch.float32, device=cuda, requires_grad=True) for _ in range(3))
+    ct = torch.tensor(1540.0, dtype=torch.float64, requires_grad=True)
+    rf = simus_rf(xt, zt, rt, ct, D, PARAMS, plan, attenuation=ATT, method=method, adjoint=adjoint)
+    assert rf.shape == (2, plan.n_fft // 2, 128) and float(rf.detach().abs().max()) == 0.0
+    loss = 0.5 * ((rf - 1.0) ** 2).sum()
+    loss.backward()
+    assert xt.grad.shape == (0,) and zt.grad.shape == (0,) and rt.grad.shape == (0,)
+    assert float(ct.grad) == 0.0
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("method", ["fp32", "mfma"])
 def test_hip_adjoint_full_frequency_directivity_matches_fp64_gradient(method, cuda):
     """Directivity at every frequency (the reference scan's branch, _simus_strategies.py:177-186): the vector
